@@ -1,0 +1,186 @@
+/*
+ * nsmi.h -- C ABI of libnsmi.so, the MI355X (gfx950) region-step engine for
+ * NetworkSolvers.jl's 2-site DMRG / TDVP sweeps.
+ *
+ * Every entry point returns an int status (NSMI_OK = 0, negative = error);
+ * nsmi_last_error(ctx) returns the message of the last failure on that context.
+ * No C++ exception crosses this boundary.  Plain pointers and sizes only.
+ *
+ * Ownership: the context owns all device memory; tensors and operators are
+ * freed explicitly (nsmi_tensor_free / nsmi_heff_destroy).  Work is enqueued
+ * on the context's HIP stream; calls that return host scalars synchronise.
+ * One host thread per context at a time; contexts are independent.
+ *
+ * Layouts (row-major, last index fastest):
+ *   site tensor  A[l, s, r]          chain MPS tensor (left link, site, right link)
+ *   MPO tensor   W[wl, wr, s', s]    s' = output (bra) index, s = input (ket) index
+ *   environment  E[bra, mpo, ket]    for every directed edge p->q (p's side seen from q)
+ *   two-site     theta[l, s1, s2, r]
+ *
+ * Reference interfaces replaced (paths relative to the NetworkSolvers.jl tree):
+ *   operator_map.jl:3-10            optimal_map(P::ProjTTN, psi)  -> nsmi_heff_create/_apply
+ *   local_solvers/eigsolve.jl:3-29  eigsolve_solver (KrylovKit)   -> nsmi_lanczos_eig
+ *   local_solvers/exponentiate.jl   exponentiate_solver           -> nsmi_krylov_expm
+ *   extracter.jl:7                  prod(psi[v] for v in region)  -> nsmi_theta
+ *   extracter.jl:14, applyexp.jl:38 itn.position (environments)   -> nsmi_env_update
+ *   extracter.jl:6, applyexp.jl:37  orthogonalize / qr            -> nsmi_qr
+ *   inserter.jl:23                  it.factorize(theta, inds; trunc...) -> nsmi_factorize
+ */
+#ifndef NSMI_H
+#define NSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NSMI_OK 0
+#define NSMI_ERR_INVALID (-1)     /* bad argument / shape mismatch            */
+#define NSMI_ERR_HIP (-2)         /* HIP runtime failure                      */
+#define NSMI_ERR_OOM (-3)         /* device pool limit or hipMalloc failure   */
+#define NSMI_ERR_SOLVER (-4)      /* rocSOLVER failure / no convergence       */
+#define NSMI_ERR_NOT_FINITE (-5)  /* NaN or Inf met in a checked input        */
+#define NSMI_ERR_UNSUPPORTED (-6) /* valid request this build does not serve  */
+
+/* element types */
+#define NSMI_F64 0
+#define NSMI_C128 1
+
+/* arithmetic of the effective-Hamiltonian contractions (the Krylov vectors,
+ * environments handed in and all reductions stay FP64) */
+#define NSMI_PREC_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4)                      */
+#define NSMI_PREC_FP32 1   /* FP32 MFMA on FP32-rounded operands                   */
+#define NSMI_PREC_BF16X3 2 /* BF16 MFMA, hi/lo split operands, 3 products, FP32 acc */
+#define NSMI_PREC_BF16 3   /* BF16 MFMA, single product, FP32 accumulation          */
+
+/* truncation routes (ITensors factorize) */
+#define NSMI_ROUTE_AUTO 0  /* "svd" if cutoff <= 1e-12, else "eigen"        */
+#define NSMI_ROUTE_SVD 1
+#define NSMI_ROUTE_EIGEN 2
+
+typedef struct nsmi_ctx nsmi_ctx;
+typedef struct nsmi_tensor nsmi_tensor;
+typedef struct nsmi_heff nsmi_heff;
+
+typedef struct {
+  int krylovdim;   /* KrylovKit krylovdim (reference default 3)          */
+  int maxiter;     /* only 1 is served (reference default)               */
+  double tol;      /* invariant-subspace / convergence tolerance (1e-14) */
+  double eta;      /* reorthogonalisation threshold, 1/sqrt(2)           */
+} nsmi_lanczos_opts;
+
+typedef struct {
+  int numops;       /* H_eff applies performed                */
+  int krylov_dim;   /* Krylov dimension reached               */
+  double normres;   /* |beta_K * y_K| residual estimate       */
+} nsmi_solver_info;
+
+typedef struct {
+  double cutoff;   /* relative discarded-weight cutoff            */
+  int64_t maxdim;  /* <= 0 means unlimited                        */
+  int64_t mindim;  /* reference default 1                         */
+  int route;       /* NSMI_ROUTE_*                                */
+} nsmi_trunc;
+
+typedef struct {
+  int64_t kept;      /* new bond dimension                        */
+  int64_t full;      /* spectrum length min(rows, cols)           */
+  double truncerr;   /* discarded weight / total weight           */
+  double total;      /* sum of the spectrum (= |theta|^2)         */
+  int route;         /* route actually taken                      */
+} nsmi_trunc_info;
+
+typedef struct {
+  int64_t launches;
+  double total_ms;   /* summed HIP-event durations                */
+  double flops;      /* summed algorithmic flops                  */
+  double bytes;      /* summed algorithmic bytes                  */
+} nsmi_profile_stats;
+
+/* kernel classes for nsmi_profile_query */
+#define NSMI_K_HEFF_GEMM1 0   /* (L . theta) GEMM of the H_eff apply           */
+#define NSMI_K_HEFF_GEMM2 1   /* (T . R) GEMM of the H_eff apply               */
+#define NSMI_K_HEFF_MIX 2     /* MPO mixing between them                       */
+#define NSMI_K_ENV 3          /* environment-update GEMMs + mixing             */
+#define NSMI_K_THETA 4        /* theta formation                               */
+#define NSMI_K_KRYLOV 5       /* Krylov vector kernels                         */
+#define NSMI_K_FACT_GEMM 6    /* Gram / projection GEMMs of factorize          */
+#define NSMI_K_FACT_SOLVE 7   /* dense eigensolver / SVD                       */
+#define NSMI_K_COUNT 8
+
+/* ---------------------------------------------------------------- context */
+int nsmi_version(void);
+int nsmi_ctx_create(int device, size_t pool_limit_bytes, nsmi_ctx** out);
+int nsmi_ctx_destroy(nsmi_ctx* ctx);
+const char* nsmi_last_error(const nsmi_ctx* ctx);
+int nsmi_sync(nsmi_ctx* ctx);
+int nsmi_pool_stats(nsmi_ctx* ctx, size_t* in_use, size_t* cached, size_t* peak);
+int nsmi_pool_trim(nsmi_ctx* ctx);
+
+/* ---------------------------------------------------------------- tensors */
+int nsmi_tensor_create(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims, nsmi_tensor** out);
+int nsmi_tensor_upload(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims, const void* host,
+                       nsmi_tensor** out);
+int nsmi_tensor_download(nsmi_ctx* ctx, const nsmi_tensor* t, void* host, size_t host_bytes);
+int nsmi_tensor_free(nsmi_ctx* ctx, nsmi_tensor* t);
+int nsmi_tensor_info(const nsmi_tensor* t, int* dtype, int* rank, int64_t* dims8);
+int nsmi_tensor_copy(nsmi_ctx* ctx, const nsmi_tensor* src, nsmi_tensor** out);
+int nsmi_tensor_reshape(nsmi_ctx* ctx, nsmi_tensor* t, int rank, const int64_t* dims);
+int nsmi_tensor_scale(nsmi_ctx* ctx, nsmi_tensor* t, double re, double im);
+int nsmi_tensor_axpy(nsmi_ctx* ctx, double re, double im, const nsmi_tensor* x, nsmi_tensor* y);
+int nsmi_tensor_norm(nsmi_ctx* ctx, const nsmi_tensor* t, double* out);
+int nsmi_tensor_inner(nsmi_ctx* ctx, const nsmi_tensor* a, const nsmi_tensor* b, double* re, double* im);
+
+/* ----------------------------------------------------------- region step */
+/* theta[l,s1,s2,r] = sum_m A[l,s1,m] B[m,s2,r]            (extracter.jl:7) */
+int nsmi_theta(nsmi_ctx* ctx, const nsmi_tensor* A, const nsmi_tensor* B, nsmi_tensor** theta);
+
+/* New environment from the previous one (itn.position, extracter.jl:14).
+ * side 0: edge i->i+1, E'[b',w',b] = sum conj(A[a',s',b']) E[a',w,a] W[w,w',s',s] A[a,s,b]
+ * side 1: edge i->i-1, E'[a',w,a]  = sum conj(A[a',s',b']) E[b',v,b] W[w,v,s',s]  A[a,s,b]
+ * precision: NSMI_PREC_* of the contractions (result stored FP64). */
+int nsmi_env_update(nsmi_ctx* ctx, const nsmi_tensor* E, const nsmi_tensor* A, const nsmi_tensor* W,
+                    int side, int precision, nsmi_tensor** out);
+
+/* Effective Hamiltonian of a 1- or 2-site chain region (optimal_map,
+ * operator_map.jl:3-10).  W2 == NULL gives the 1-site operator.  The operator
+ * keeps references to L, W1, W2, R (they must outlive it) plus its own
+ * workspace and precision-converted copies. */
+int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1, const nsmi_tensor* W2,
+                     const nsmi_tensor* R, int precision, nsmi_heff** out);
+int nsmi_heff_destroy(nsmi_ctx* ctx, nsmi_heff* op);
+int nsmi_heff_apply(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y);
+/* FP64 Rayleigh quotient <x|H|x>/<x|x> with an FP64 apply whatever op's precision */
+int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, double* e);
+
+/* Lowest Ritz pair (KrylovKit eigsolve with Lanczos, ishermitian, :SR;
+ * local_solvers/eigsolve.jl:3-29).  x is the start vector on entry and the
+ * Ritz vector on exit. */
+int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_lanczos_opts* opts,
+                     double* eigval, nsmi_solver_info* info);
+
+/* Truncated split of theta (ITensors factorize, ortho="left"; inserter.jl:23).
+ * rows = the first n_row_dims dims of theta when rows_trailing == 0, else the
+ * remaining trailing dims.  The isometry U carries the row dims:
+ *   rows_trailing == 0: U[dims[:n], k],   C[k, dims[n:]]
+ *   rows_trailing == 1: U[k, dims[n:]],   C[dims[:n], k]   (U stored transposed)
+ * Columns of the isometry get the "largest-magnitude entry real positive" gauge. */
+int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing,
+                   const nsmi_trunc* trunc, nsmi_tensor** U, nsmi_tensor** C, nsmi_trunc_info* info);
+
+/* Thin QR with non-negative diag(R) (ITensors qr in the orthogonalize walk,
+ * extracter.jl:6).  Same row/transpose conventions as nsmi_factorize. */
+int nsmi_qr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Q,
+            nsmi_tensor** R);
+
+/* ------------------------------------------------------------- profiling */
+int nsmi_profile_enable(nsmi_ctx* ctx, int on);
+int nsmi_profile_reset(nsmi_ctx* ctx);
+int nsmi_profile_query(nsmi_ctx* ctx, int kernel_class, nsmi_profile_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSMI_H */

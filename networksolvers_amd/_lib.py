@@ -1,0 +1,117 @@
+"""ctypes binding of libnsmi.so (the C ABI declared in include/nsmi.h).
+
+The library is loaded from the package directory.  There is no fallback: when
+the shared object is missing or fails to load, ``load()`` raises, so nothing in
+this package can silently run the region step anywhere but on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsmi.so")
+
+NSMI_OK = 0
+ERRORS = {-1: "NSMI_ERR_INVALID", -2: "NSMI_ERR_HIP", -3: "NSMI_ERR_OOM", -4: "NSMI_ERR_SOLVER",
+          -5: "NSMI_ERR_NOT_FINITE", -6: "NSMI_ERR_UNSUPPORTED"}
+
+F64, C128 = 0, 1
+PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
+ROUTE = {"auto": 0, "svd": 1, "eigen": 2}
+ROUTE_NAME = {1: "svd", 2: "eigen"}
+K_CLASSES = {"heff_gemm1": 0, "heff_gemm2": 1, "heff_mix": 2, "env": 3, "theta": 4, "krylov": 5,
+             "fact_gemm": 6, "fact_solve": 7}
+
+
+class LanczosOpts(C.Structure):
+    _fields_ = [("krylovdim", C.c_int), ("maxiter", C.c_int), ("tol", C.c_double), ("eta", C.c_double)]
+
+
+class SolverInfo(C.Structure):
+    _fields_ = [("numops", C.c_int), ("krylov_dim", C.c_int), ("normres", C.c_double)]
+
+
+class Trunc(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("maxdim", C.c_int64), ("mindim", C.c_int64), ("route", C.c_int)]
+
+
+class TruncInfo(C.Structure):
+    _fields_ = [("kept", C.c_int64), ("full", C.c_int64), ("truncerr", C.c_double), ("total", C.c_double),
+                ("route", C.c_int)]
+
+
+class ProfileStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("flops", C.c_double), ("bytes", C.c_double)]
+
+
+class NsmiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_VP = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_I64P = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "nsmi_version": (C.c_int, []),
+    "nsmi_ctx_create": (C.c_int, [C.c_int, C.c_size_t, _PP]),
+    "nsmi_ctx_destroy": (C.c_int, [_VP]),
+    "nsmi_last_error": (C.c_char_p, [_VP]),
+    "nsmi_sync": (C.c_int, [_VP]),
+    "nsmi_pool_stats": (C.c_int, [_VP, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "nsmi_pool_trim": (C.c_int, [_VP]),
+    "nsmi_tensor_create": (C.c_int, [_VP, C.c_int, C.c_int, _I64P, _PP]),
+    "nsmi_tensor_upload": (C.c_int, [_VP, C.c_int, C.c_int, _I64P, _VP, _PP]),
+    "nsmi_tensor_download": (C.c_int, [_VP, _VP, _VP, C.c_size_t]),
+    "nsmi_tensor_free": (C.c_int, [_VP, _VP]),
+    "nsmi_tensor_info": (C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), _I64P]),
+    "nsmi_tensor_copy": (C.c_int, [_VP, _VP, _PP]),
+    "nsmi_tensor_reshape": (C.c_int, [_VP, _VP, C.c_int, _I64P]),
+    "nsmi_tensor_scale": (C.c_int, [_VP, _VP, C.c_double, C.c_double]),
+    "nsmi_tensor_axpy": (C.c_int, [_VP, C.c_double, C.c_double, _VP, _VP]),
+    "nsmi_tensor_norm": (C.c_int, [_VP, _VP, C.POINTER(C.c_double)]),
+    "nsmi_tensor_inner": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "nsmi_theta": (C.c_int, [_VP, _VP, _VP, _PP]),
+    "nsmi_env_update": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, C.c_int, _PP]),
+    "nsmi_heff_create": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, _PP]),
+    "nsmi_heff_destroy": (C.c_int, [_VP, _VP]),
+    "nsmi_heff_apply": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "nsmi_heff_rayleigh": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_double)]),
+    "nsmi_lanczos_eig": (C.c_int, [_VP, _VP, _VP, C.POINTER(LanczosOpts), C.POINTER(C.c_double),
+                                   C.POINTER(SolverInfo)]),
+    "nsmi_factorize": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.POINTER(Trunc), _PP, _PP, C.POINTER(TruncInfo)]),
+    "nsmi_qr": (C.c_int, [_VP, _VP, C.c_int, C.c_int, _PP, _PP]),
+    "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),
+    "nsmi_profile_reset": (C.c_int, [_VP]),
+    "nsmi_profile_query": (C.c_int, [_VP, C.c_int, C.POINTER(ProfileStats)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libnsmi.so once and declare every signature; raises when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                          f"g.build()'` (or `make -C networksolvers_amd/csrc`)")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(ctx_handle, code):
+    if code != NSMI_OK:
+        lib = load()
+        msg = lib.nsmi_last_error(ctx_handle) if ctx_handle else b""
+        raise NsmiError(code, (msg or b"").decode(errors="replace"))

@@ -1,0 +1,392 @@
+// Context, device pool, tensor handles and profiling of libnsmi.so.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+using namespace nsmi;
+
+// ------------------------------------------------------------------ pool
+size_t Pool::round(size_t b) {
+  if (b == 0) b = 1;
+  if (b <= (size_t(1) << 20)) {
+    size_t r = 512;
+    while (r < b) r <<= 1;
+    return r;
+  }
+  const size_t g = size_t(2) << 20;
+  return (b + g - 1) / g * g;
+}
+
+void* Pool::alloc(size_t bytes) {
+  size_t r = round(bytes);
+  auto it = free_.find(r);
+  if (it != free_.end() && !it->second.empty()) {
+    void* p = it->second.back();
+    it->second.pop_back();
+    cached_ -= r;
+    in_use_ += r;
+    if (in_use_ > peak_) peak_ = in_use_;
+    live_[p] = r;
+    return p;
+  }
+  if (limit_ && in_use_ + cached_ + r > limit_) trim();
+  if (limit_ && in_use_ + r > limit_)
+    fail(NSMI_ERR_OOM, "device pool limit exceeded: need " + std::to_string(r) + " B, in use " +
+                           std::to_string(in_use_) + " B, limit " + std::to_string(limit_) + " B");
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, r);
+  if (e != hipSuccess) {
+    trim();
+    e = hipMalloc(&p, r);
+    if (e != hipSuccess) fail(NSMI_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  in_use_ += r;
+  if (in_use_ > peak_) peak_ = in_use_;
+  live_[p] = r;
+  return p;
+}
+
+void Pool::release(void* p) {
+  if (!p) return;
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  size_t r = it->second;
+  live_.erase(it);
+  in_use_ -= r;
+  cached_ += r;
+  free_[r].push_back(p);
+}
+
+void Pool::trim() {
+  // callers guarantee the stream is idle or that released blocks are no longer read
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_)
+    for (void* p : kv.second) (void)hipFree(p);
+  free_.clear();
+  cached_ = 0;
+}
+
+Pool::~Pool() {
+  trim();
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  live_.clear();
+}
+
+// --------------------------------------------------------------- helpers
+namespace nsmi {
+
+nsmi_tensor* tensor_new(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims) {
+  require(rank >= 0 && rank <= 8, "tensor rank must be in [0, 8]");
+  auto* t = new nsmi_tensor();
+  t->dtype = dtype;
+  t->rank = rank;
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) {
+    if (dims[i] < 0) {
+      delete t;
+      fail(NSMI_ERR_INVALID, "negative tensor dimension");
+    }
+    t->dims[i] = dims[i];
+    n *= dims[i];
+  }
+  for (int i = rank; i < 8; ++i) t->dims[i] = 1;
+  t->numel = n;
+  t->bytes = size_t(n) * dtype_size(dtype);
+  try {
+    t->data = ctx->pool->alloc(t->bytes ? t->bytes : 1);
+  } catch (...) {
+    delete t;
+    throw;
+  }
+  return t;
+}
+
+void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t) {
+  if (!t) return;
+  ctx->pool->release(t->data);
+  delete t;
+}
+
+void* scratch(nsmi_ctx* ctx, size_t bytes) { return ctx->pool->alloc(bytes); }
+void scratch_release(nsmi_ctx* ctx, void* p) { ctx->pool->release(p); }
+
+static hipEvent_t take_event(nsmi_ctx* ctx) {
+  if (!ctx->spare_events.empty()) {
+    hipEvent_t e = ctx->spare_events.back();
+    ctx->spare_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  NSMI_HIP(hipEventCreate(&e));
+  return e;
+}
+
+ProfScope::ProfScope(nsmi_ctx* c, int k, double f, double by) : ctx(c), cls(k), flops(f), bytes(by) {
+  if (!ctx->prof_on || k < 0) return;
+  a = take_event(ctx);
+  b = take_event(ctx);
+  NSMI_HIP(hipEventRecord(a, ctx->stream));
+}
+
+ProfScope::~ProfScope() {
+  if (!a) return;
+  if (hipEventRecord(b, ctx->stream) != hipSuccess) return;
+  ctx->recs.push_back(ProfRec{a, b, cls, flops, bytes});
+}
+
+}  // namespace nsmi
+
+static void drain_profile(nsmi_ctx* ctx) {
+  if (ctx->recs.empty()) return;
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    float ms = 0.f;
+    NSMI_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto& s = ctx->stats[r.cls];
+    s.launches += 1;
+    s.total_ms += ms;
+    s.flops += r.flops;
+    s.bytes += r.bytes;
+    ctx->spare_events.push_back(r.a);
+    ctx->spare_events.push_back(r.b);
+  }
+  ctx->recs.clear();
+}
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                               \
+  }                                                                 \
+  catch (const nsmi::Error& e) {                                    \
+    if (ctxp) (ctxp)->err = e.what();                               \
+    return e.code;                                                  \
+  }                                                                 \
+  catch (const std::exception& e) {                                 \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what();   \
+    return NSMI_ERR_INVALID;                                        \
+  }                                                                 \
+  return NSMI_OK;
+
+extern "C" {
+
+int nsmi_version(void) { return 10000; }
+
+int nsmi_ctx_create(int device, size_t pool_limit_bytes, nsmi_ctx** out) {
+  if (!out) return NSMI_ERR_INVALID;
+  *out = nullptr;
+  nsmi_ctx* ctx = new nsmi_ctx();
+  try {
+    int n = 0;
+    NSMI_HIP(hipGetDeviceCount(&n));
+    require(device >= 0 && device < n, "device index out of range");
+    ctx->device = device;
+    NSMI_HIP(hipSetDevice(device));
+    NSMI_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->pool = new Pool(pool_limit_bytes);
+    NSMI_HIP(hipMalloc(&ctx->d_partials, sizeof(double) * kPartials * 4));
+    NSMI_HIP(hipMalloc(&ctx->d_scal, sizeof(double) * 64));
+    NSMI_HIP(hipHostMalloc(&ctx->h_scal, sizeof(double) * 64, hipHostMallocDefault));
+    NSMI_HIP(hipMalloc(&ctx->d_info, sizeof(int) * 4));
+    if (rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+      fail(NSMI_ERR_SOLVER, "rocblas_create_handle failed");
+    if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+      fail(NSMI_ERR_SOLVER, "rocblas_set_stream failed");
+  } catch (const nsmi::Error& e) {
+    fprintf(stderr, "nsmi_ctx_create: %s\n", e.what());
+    nsmi_ctx_destroy(ctx);
+    return e.code;
+  }
+  *out = ctx;
+  return NSMI_OK;
+}
+
+int nsmi_ctx_destroy(nsmi_ctx* ctx) {
+  if (!ctx) return NSMI_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+  delete ctx->pool;
+  if (ctx->d_partials) (void)hipFree(ctx->d_partials);
+  if (ctx->d_scal) (void)hipFree(ctx->d_scal);
+  if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  if (ctx->d_info) (void)hipFree(ctx->d_info);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return NSMI_OK;
+}
+
+const char* nsmi_last_error(const nsmi_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int nsmi_sync(nsmi_ctx* ctx) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  API_END(ctx)
+}
+
+int nsmi_pool_stats(nsmi_ctx* ctx, size_t* in_use, size_t* cached, size_t* peak) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  if (in_use) *in_use = ctx->pool->in_use();
+  if (cached) *cached = ctx->pool->cached();
+  if (peak) *peak = ctx->pool->peak();
+  return NSMI_OK;
+}
+
+int nsmi_pool_trim(nsmi_ctx* ctx) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->pool->trim();
+  API_END(ctx)
+}
+
+int nsmi_tensor_create(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims, nsmi_tensor** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(rank == 0 || dims, "dims is null");
+  *out = tensor_new(ctx, dtype, rank, dims);
+  API_END(ctx)
+}
+
+int nsmi_tensor_upload(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims, const void* host,
+                       nsmi_tensor** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(rank == 0 || dims, "dims is null");
+  nsmi_tensor* t = tensor_new(ctx, dtype, rank, dims);
+  if (t->bytes) {
+    if (!host) {
+      tensor_delete(ctx, t);
+      fail(NSMI_ERR_INVALID, "host pointer is null");
+    }
+    NSMI_HIP(hipMemcpyAsync(t->data, host, t->bytes, hipMemcpyHostToDevice, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));  // the caller may reuse its buffer on return
+  }
+  *out = t;
+  API_END(ctx)
+}
+
+int nsmi_tensor_download(nsmi_ctx* ctx, const nsmi_tensor* t, void* host, size_t host_bytes) {
+  if (!ctx || !t) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(host_bytes >= t->bytes, "host buffer too small for download");
+  if (t->bytes) {
+    NSMI_HIP(hipMemcpyAsync(host, t->data, t->bytes, hipMemcpyDeviceToHost, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  API_END(ctx)
+}
+
+int nsmi_tensor_free(nsmi_ctx* ctx, nsmi_tensor* t) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  tensor_delete(ctx, t);
+  API_END(ctx)
+}
+
+int nsmi_tensor_info(const nsmi_tensor* t, int* dtype, int* rank, int64_t* dims8) {
+  if (!t) return NSMI_ERR_INVALID;
+  if (dtype) *dtype = t->dtype;
+  if (rank) *rank = t->rank;
+  if (dims8)
+    for (int i = 0; i < 8; ++i) dims8[i] = t->dims[i];
+  return NSMI_OK;
+}
+
+int nsmi_tensor_copy(nsmi_ctx* ctx, const nsmi_tensor* src, nsmi_tensor** out) {
+  if (!ctx || !src || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  nsmi_tensor* t = tensor_new(ctx, src->dtype, src->rank, src->dims);
+  if (t->bytes)
+    NSMI_HIP(hipMemcpyAsync(t->data, src->data, t->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  *out = t;
+  API_END(ctx)
+}
+
+int nsmi_tensor_reshape(nsmi_ctx* ctx, nsmi_tensor* t, int rank, const int64_t* dims) {
+  if (!ctx || !t) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(rank >= 0 && rank <= 8, "rank must be in [0, 8]");
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) n *= dims[i];
+  require(n == t->numel, "reshape must preserve the element count");
+  t->rank = rank;
+  for (int i = 0; i < 8; ++i) t->dims[i] = i < rank ? dims[i] : 1;
+  API_END(ctx)
+}
+
+int nsmi_tensor_scale(nsmi_ctx* ctx, nsmi_tensor* t, double re, double im) {
+  if (!ctx || !t) return NSMI_ERR_INVALID;
+  API_BEGIN
+  if (t->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "scale: only F64 tensors");
+  require(im == 0.0, "scale: imaginary factor on a real tensor");
+  vec_scale(ctx, static_cast<double*>(t->data), t->numel, re);
+  API_END(ctx)
+}
+
+int nsmi_tensor_axpy(nsmi_ctx* ctx, double re, double im, const nsmi_tensor* x, nsmi_tensor* y) {
+  if (!ctx || !x || !y) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(x->numel == y->numel && x->dtype == y->dtype, "axpy: size/dtype mismatch");
+  if (x->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "axpy: only F64 tensors");
+  require(im == 0.0, "axpy: imaginary factor on a real tensor");
+  vec_axpy(ctx, static_cast<double*>(y->data), static_cast<const double*>(x->data), x->numel, re);
+  API_END(ctx)
+}
+
+int nsmi_tensor_norm(nsmi_ctx* ctx, const nsmi_tensor* t, double* out) {
+  if (!ctx || !t || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  if (t->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "norm: only F64 tensors");
+  const double* x = static_cast<const double*>(t->data);
+  vec_dot_dev(ctx, x, x, t->numel, 0);
+  double v;
+  read_slots(ctx, 1, &v);
+  *out = std::sqrt(v);
+  API_END(ctx)
+}
+
+int nsmi_tensor_inner(nsmi_ctx* ctx, const nsmi_tensor* a, const nsmi_tensor* b, double* re, double* im) {
+  if (!ctx || !a || !b) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(a->numel == b->numel && a->dtype == b->dtype, "inner: size/dtype mismatch");
+  if (a->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "inner: only F64 tensors");
+  vec_dot_dev(ctx, static_cast<const double*>(a->data), static_cast<const double*>(b->data), a->numel, 0);
+  double v;
+  read_slots(ctx, 1, &v);
+  if (re) *re = v;
+  if (im) *im = 0.0;
+  API_END(ctx)
+}
+
+int nsmi_profile_enable(nsmi_ctx* ctx, int on) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  if (!on) drain_profile(ctx);
+  ctx->prof_on = on != 0;
+  API_END(ctx)
+}
+
+int nsmi_profile_reset(nsmi_ctx* ctx) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  drain_profile(ctx);
+  for (auto& s : ctx->stats) s = nsmi_profile_stats{};
+  API_END(ctx)
+}
+
+int nsmi_profile_query(nsmi_ctx* ctx, int kernel_class, nsmi_profile_stats* out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(kernel_class >= 0 && kernel_class < NSMI_K_COUNT, "unknown kernel class");
+  drain_profile(ctx);
+  *out = ctx->stats[kernel_class];
+  API_END(ctx)
+}
+
+}  // extern "C"

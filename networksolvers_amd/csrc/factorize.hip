@@ -1,0 +1,495 @@
+// Truncated factorisation theta = U C (ITensors factorize, inserter.jl:23) and the
+// QR gauge walk (extracter.jl:6).  FP64 throughout.
+//
+// "eigen" route: rho = M M^T by MFMA GEMM, dense symmetric eigensolver
+// (rocSOLVER dsyevd), spectrum sorted by decreasing |lambda| on the host,
+// NDTensors truncate! restated on the host, then the kept eigenvectors are
+// gathered with the gauge fix and C = U^T M by MFMA GEMM.
+// "svd" route: rocSOLVER dgesvd of M, P = s^2, same truncation and gauge.
+// rocSOLVER is column-major: a row-major m x n buffer is its n x m transpose.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "internal.h"
+
+using namespace nsmi;
+
+namespace nsmi {
+namespace {
+
+// restatement of NDTensors truncate! (relative cutoff); returns kept count
+int64_t truncate_spectrum(std::vector<double>& P, int64_t mindim, int64_t maxdim, double cutoff, double* truncerr,
+                          double* total) {
+  const int64_t origm = int64_t(P.size());
+  *truncerr = 0.0;
+  *total = 0.0;
+  if (origm == 0) return 0;
+  if (origm == 1) {
+    *total = P[0];
+    return 1;
+  }
+  if (P[0] < 0)
+    for (auto& v : P) v = -v;
+  for (int64_t n = origm - 1; n >= 0; --n) {
+    if (P[n] >= 0) break;
+    P[n] = 0.0;
+  }
+  int64_t n = origm;
+  double err = 0.0;
+  while (n > maxdim) {
+    err += P[n - 1];
+    --n;
+  }
+  double scale = 0.0;
+  for (double v : P) scale += v;
+  *total = scale;
+  if (scale == 0.0) scale = 1.0;
+  while (n > mindim && (err + P[n - 1] <= cutoff * scale)) {
+    err += P[n - 1];
+    --n;
+  }
+  err /= scale;
+  if (n < 1) n = 1;
+  *truncerr = err;
+  return n;
+}
+
+// For each of k vectors (row r of a row-major matrix with row stride ld when
+// rows == 1, else column c with column stride 1 and row stride ld), find the
+// first index of the largest magnitude and write its sign (+1/-1) to sgn.
+__global__ void k_gauge_sign(const double* __restrict__ X, int64_t len, int64_t ld, int rows,
+                             const int* __restrict__ idx, double* __restrict__ sgn) {
+  const int j = blockIdx.x;
+  const int64_t base = rows ? int64_t(idx ? idx[j] : j) * ld : int64_t(idx ? idx[j] : j);
+  const int64_t step = rows ? 1 : ld;
+  double best = -1.0;
+  int64_t bi = 0;
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const double a = fabs(X[base + i * step]);
+    if (a > best) {  // strict: first index wins within a thread's stride
+      best = a;
+      bi = i;
+    }
+  }
+  __shared__ double sb[256];
+  __shared__ int64_t si[256];
+  sb[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double b2 = sb[threadIdx.x + o];
+      const int64_t i2 = si[threadIdx.x + o];
+      if (b2 > sb[threadIdx.x] || (b2 == sb[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sb[threadIdx.x] = b2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sgn[j] = (X[base + si[0] * step] < 0.0) ? -1.0 : 1.0;
+}
+
+// out[j, i] = scale[j] * sgn[j] * X[idx[j], i]   (gather rows; out row-major k x len)
+__global__ void k_gather_rows(const double* __restrict__ X, int64_t ld, const int* __restrict__ idx, int k,
+                              int64_t len, const double* __restrict__ sgn, const double* __restrict__ scale,
+                              double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(k) * len) return;
+  const int j = int(t / len);
+  const int64_t i = t % len;
+  const int src = idx ? idx[j] : j;
+  double f = sgn[j];
+  if (scale) f *= scale[j];
+  out[t] = f * X[int64_t(src) * ld + i];
+}
+
+// out[i, j] = scale[j] * sgn[j] * X[i, idx[j]]   (gather columns into row-major len x k)
+__global__ void k_gather_cols(const double* __restrict__ X, int64_t ld, const int* __restrict__ idx, int k,
+                              int64_t len, const double* __restrict__ sgn, const double* __restrict__ scale,
+                              double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(k) * len) return;
+  const int64_t i = t / k;
+  const int j = int(t % k);
+  const int src = idx ? idx[j] : j;
+  double f = sgn[j];
+  if (scale) f *= scale[j];
+  out[t] = f * X[i * ld + src];
+}
+
+// out (len x k, row-major) = transpose of in (k x len, row-major)
+__global__ void k_transpose(const double* __restrict__ in, int k, int64_t len, double* __restrict__ out) {
+  __shared__ double tile[32][33];
+  const int64_t i0 = int64_t(blockIdx.x) * 32, j0 = int64_t(blockIdx.y) * 32;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int64_t j = j0 + r, i = i0 + threadIdx.x;
+    if (j < k && i < len) tile[r][threadIdx.x] = in[j * len + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int64_t i = i0 + r, j = j0 + threadIdx.x;
+    if (i < len && j < k) out[i * k + j] = tile[threadIdx.x][r];
+  }
+}
+
+// R (r x n row-major) from a gelqf/geqrf output read row-major, zeroing the
+// triangle below (upper == 1 keeps j >= i) and applying row signs.
+__global__ void k_extract_tri(const double* __restrict__ X, int64_t ldx, int64_t r, int64_t n,
+                              const double* __restrict__ sgn, double* __restrict__ out, int transpose_out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= r * n) return;
+  const int64_t i = t / n, j = t % n;
+  const double v = (j >= i) ? sgn[i] * X[i * ldx + j] : 0.0;
+  if (transpose_out)
+    out[j * r + i] = v;
+  else
+    out[t] = v;
+}
+
+// R^T (m x r row-major) from a geqrf output read row-major (m x ldx): element
+// (j, i) = sgn[i] * X[j*ldx + i] for i <= j, else 0.
+__global__ void k_extract_tri_t(const double* __restrict__ X, int64_t ldx, int64_t r, int64_t m,
+                                const double* __restrict__ sgn, double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= r * m) return;
+  const int64_t j = t / r, i = t % r;
+  out[t] = (i <= j) ? sgn[i] * X[j * ldx + i] : 0.0;
+}
+
+// Q columns (or rows) times the diagonal signs of R
+__global__ void k_scale_cols(double* __restrict__ X, int64_t rows, int64_t cols, const double* __restrict__ sgn) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  X[t] *= sgn[t % cols];
+}
+__global__ void k_scale_rows(double* __restrict__ X, int64_t rows, int64_t cols, const double* __restrict__ sgn) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  X[t] *= sgn[t / cols];
+}
+// sign of the diagonal entries of a row-major matrix (row stride ld)
+__global__ void k_diag_sign(const double* __restrict__ X, int64_t ld, int64_t r, double* __restrict__ sgn) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < r) sgn[i] = X[i * ld + i] < 0.0 ? -1.0 : 1.0;
+}
+
+inline unsigned blocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
+
+GemmDesc gdesc(int64_t M, int64_t N, int64_t K, bool ta, bool tb) {
+  GemmDesc g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.transA = ta;
+  g.transB = tb;
+  g.lda = ta ? M : K;
+  g.ldb = tb ? K : N;
+  g.ldc = N;
+  g.batch = 1;
+  return g;
+}
+
+void check_rocsolver(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success)
+    fail(NSMI_ERR_SOLVER, std::string(what) + " failed with rocblas_status " + std::to_string(int(st)));
+}
+
+void check_info(nsmi_ctx* ctx, const char* what) {
+  int info = 0;
+  NSMI_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  if (info != 0) fail(NSMI_ERR_SOLVER, std::string(what) + " returned info = " + std::to_string(info));
+}
+
+}  // namespace
+}  // namespace nsmi
+
+extern "C" {
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                             \
+  }                                                               \
+  catch (const nsmi::Error& e) {                                  \
+    if (ctxp) (ctxp)->err = e.what();                             \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what(); \
+    return NSMI_ERR_INVALID;                                      \
+  }                                                               \
+  return NSMI_OK;
+
+int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing,
+                   const nsmi_trunc* trunc, nsmi_tensor** Uout, nsmi_tensor** Cout, nsmi_trunc_info* info) {
+  if (!ctx || !theta || !trunc || !Uout || !Cout) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *Uout = nullptr;
+  *Cout = nullptr;
+  if (theta->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "factorize: only F64 tensors are served");
+  require(n_row_dims >= 1 && n_row_dims < theta->rank, "factorize: n_row_dims must split theta's dims");
+  int64_t m = 1, nc = 1;  // M = theta viewed as m x nc (leading dims x trailing dims)
+  for (int i = 0; i < n_row_dims; ++i) m *= theta->dims[i];
+  for (int i = n_row_dims; i < theta->rank; ++i) nc *= theta->dims[i];
+  require(m >= 1 && nc >= 1, "factorize: empty tensor");
+  require(m < (int64_t(1) << 31) && nc < (int64_t(1) << 31), "factorize: matrix too large");
+  // the isometry side: rows of M (rows_trailing == 0) or columns of M
+  const int64_t iso = rows_trailing ? nc : m, oth = rows_trailing ? m : nc;
+  const int64_t full = std::min(m, nc);
+  int64_t maxdim = trunc->maxdim > 0 ? std::min<int64_t>(trunc->maxdim, full) : full;
+  const int64_t mindim = trunc->mindim > 0 ? trunc->mindim : 1;
+  int route = trunc->route;
+  if (route == NSMI_ROUTE_AUTO) route = (trunc->cutoff <= 1e-12) ? NSMI_ROUTE_SVD : NSMI_ROUTE_EIGEN;
+  require(route == NSMI_ROUTE_SVD || route == NSMI_ROUTE_EIGEN, "factorize: unknown route");
+  const double* M = static_cast<const double*>(theta->data);
+
+  std::vector<double> P;
+  std::vector<int> order;
+  int64_t k = 0;
+  double truncerr = 0.0, total = 0.0;
+  nsmi_tensor *U = nullptr, *C = nullptr;
+  Scratch d_sgn(ctx, sizeof(double) * std::max<int64_t>(full, 1));
+  Scratch d_idx(ctx, sizeof(int) * std::max<int64_t>(full, 1));
+
+  int64_t udims[8], cdims[8];
+  int ur = 0, cr = 0;
+  auto set_dims = [&](int64_t kk) {
+    ur = cr = 0;
+    if (!rows_trailing) {
+      for (int i = 0; i < n_row_dims; ++i) udims[ur++] = theta->dims[i];
+      udims[ur++] = kk;
+      cdims[cr++] = kk;
+      for (int i = n_row_dims; i < theta->rank; ++i) cdims[cr++] = theta->dims[i];
+    } else {
+      udims[ur++] = kk;
+      for (int i = n_row_dims; i < theta->rank; ++i) udims[ur++] = theta->dims[i];
+      for (int i = 0; i < n_row_dims; ++i) cdims[cr++] = theta->dims[i];
+      cdims[cr++] = kk;
+    }
+  };
+
+  if (route == NSMI_ROUTE_EIGEN) {
+    // rho over the isometry side (iso x iso)
+    Scratch rho(ctx, sizeof(double) * size_t(iso) * iso);
+    Scratch w(ctx, sizeof(double) * size_t(iso));
+    Scratch e(ctx, sizeof(double) * size_t(iso));
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
+      if (!rows_trailing)
+        gemm_f64(ctx, gdesc(m, m, nc, false, true), 1.0, M, M, 0.0, rho.as<double>(), kRoleFact);
+      else
+        gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, M, M, 0.0, rho.as<double>(), kRoleFact);
+    }
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
+      check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
+                                       rho.as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(), ctx->d_info),
+                      "rocsolver_dsyevd");
+    }
+    check_info(ctx, "rocsolver_dsyevd");
+    std::vector<double> wh(iso);
+    NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    for (double v : wh)
+      if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
+    order.resize(iso);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
+    P.resize(iso);
+    for (int64_t i = 0; i < iso; ++i) P[i] = wh[order[i]];
+    k = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
+    NSMI_HIP(hipMemcpyAsync(d_idx.p, order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
+    hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, rho.as<double>(), iso, iso, 1,
+                       d_idx.as<int>(), d_sgn.as<double>());
+    NSMI_LAUNCH_CHECK();
+    Scratch Ut(ctx, sizeof(double) * size_t(k) * iso);
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, rho.as<double>(), iso,
+                       d_idx.as<int>(), int(k), iso, d_sgn.as<double>(), (const double*)nullptr, Ut.as<double>());
+    NSMI_LAUNCH_CHECK();
+    set_dims(k);
+    U = tensor_new(ctx, NSMI_F64, ur, udims);
+    C = tensor_new(ctx, NSMI_F64, cr, cdims);
+    ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(k) * m * nc, 8.0 * (double(m) * nc + double(k) * (m + nc)));
+    if (!rows_trailing) {
+      // U (m x k) = Ut^T ; C (k x nc) = Ut . M
+      hipLaunchKernelGGL(k_transpose, dim3(blocks(m, 32), blocks(k, 32)), dim3(32, 8), 0, ctx->stream, Ut.as<double>(),
+                         int(k), m, static_cast<double*>(U->data));
+      NSMI_LAUNCH_CHECK();
+      gemm_f64(ctx, gdesc(k, nc, m, false, false), 1.0, Ut.as<double>(), M, 0.0, static_cast<double*>(C->data),
+               kRoleFact);
+    } else {
+      // U (k x nc) = Ut ; C (m x k) = M . Ut^T
+      NSMI_HIP(hipMemcpyAsync(U->data, Ut.p, sizeof(double) * k * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      gemm_f64(ctx, gdesc(m, k, nc, false, true), 1.0, M, Ut.as<double>(), 0.0, static_cast<double*>(C->data),
+               kRoleFact);
+    }
+  } else {
+    // SVD of the column-major view M^T (nc x m): M^T = Ut S Vt  =>  M = Vt^T S Ut^T
+    // row-major readings: buffer "vt" (full x m col-major) = left vectors of M as m x full rows;
+    // buffer "u" (nc x full col-major) = right vectors of M as full x nc rows.
+    Scratch a(ctx, sizeof(double) * size_t(m) * nc);
+    Scratch s(ctx, sizeof(double) * size_t(full));
+    Scratch ub(ctx, sizeof(double) * size_t(nc) * full);
+    Scratch vb(ctx, sizeof(double) * size_t(full) * m);
+    Scratch e(ctx, sizeof(double) * size_t(std::max<int64_t>(full, 2)));
+    NSMI_HIP(hipMemcpyAsync(a.p, M, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
+      check_rocsolver(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_singular, rocblas_int(nc),
+                                       rocblas_int(m), a.as<double>(), rocblas_int(nc), s.as<double>(),
+                                       ub.as<double>(), rocblas_int(nc), vb.as<double>(), rocblas_int(full),
+                                       e.as<double>(), rocblas_outofplace, ctx->d_info),
+                      "rocsolver_dgesvd");
+    }
+    check_info(ctx, "rocsolver_dgesvd");
+    std::vector<double> sh(full);
+    NSMI_HIP(hipMemcpyAsync(sh.data(), s.p, sizeof(double) * full, hipMemcpyDeviceToHost, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    P.resize(full);
+    for (int64_t i = 0; i < full; ++i) {
+      if (!std::isfinite(sh[i])) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite singular value");
+      P[i] = sh[i] * sh[i];
+    }
+    k = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
+    set_dims(k);
+    U = tensor_new(ctx, NSMI_F64, ur, udims);
+    C = tensor_new(ctx, NSMI_F64, cr, cdims);
+    if (!rows_trailing) {
+      // isometry = left vectors (columns of the m x full row-major vb)
+      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, vb.as<double>(), m, full, 0,
+                         (const int*)nullptr, d_sgn.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, vb.as<double>(), full,
+                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), (const double*)nullptr,
+                         static_cast<double*>(U->data));
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, ub.as<double>(), nc,
+                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), s.as<double>(),
+                         static_cast<double*>(C->data));
+      NSMI_LAUNCH_CHECK();
+    } else {
+      // isometry = right vectors (rows of the full x nc row-major ub)
+      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, ub.as<double>(), nc, nc, 1,
+                         (const int*)nullptr, d_sgn.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, ub.as<double>(), nc,
+                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), (const double*)nullptr,
+                         static_cast<double*>(U->data));
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, vb.as<double>(), full,
+                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), s.as<double>(),
+                         static_cast<double*>(C->data));
+      NSMI_LAUNCH_CHECK();
+    }
+  }
+  *Uout = U;
+  *Cout = C;
+  if (info) {
+    info->kept = k;
+    info->full = full;
+    info->truncerr = truncerr;
+    info->total = total;
+    info->route = route;
+  }
+  // scratch blocks are returned to the pool in stream order; the host vectors
+  // copied above were synchronised already
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  API_END(ctx)
+}
+
+int nsmi_qr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Qout,
+            nsmi_tensor** Rout) {
+  if (!ctx || !A || !Qout || !Rout) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *Qout = nullptr;
+  *Rout = nullptr;
+  if (A->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "qr: only F64 tensors are served");
+  require(n_row_dims >= 1 && n_row_dims < A->rank, "qr: n_row_dims must split A's dims");
+  int64_t m = 1, nc = 1;
+  for (int i = 0; i < n_row_dims; ++i) m *= A->dims[i];
+  for (int i = n_row_dims; i < A->rank; ++i) nc *= A->dims[i];
+  // factor X = Q R where X = M (rows_trailing == 0) or M^T
+  const int64_t xr = rows_trailing ? nc : m, xc = rows_trailing ? m : nc;
+  const int64_t r = std::min(xr, xc);
+  Scratch a(ctx, sizeof(double) * size_t(m) * nc);
+  Scratch tau(ctx, sizeof(double) * size_t(std::max<int64_t>(r, 1)));
+  Scratch sg(ctx, sizeof(double) * size_t(std::max<int64_t>(r, 1)));
+  NSMI_HIP(hipMemcpyAsync(a.p, A->data, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  int64_t qd[8], rd[8];
+  int qr = 0, rr = 0;
+  if (!rows_trailing) {
+    for (int i = 0; i < n_row_dims; ++i) qd[qr++] = A->dims[i];
+    qd[qr++] = r;
+    rd[rr++] = r;
+    for (int i = n_row_dims; i < A->rank; ++i) rd[rr++] = A->dims[i];
+  } else {
+    qd[qr++] = r;
+    for (int i = n_row_dims; i < A->rank; ++i) qd[qr++] = A->dims[i];
+    for (int i = 0; i < n_row_dims; ++i) rd[rr++] = A->dims[i];
+    rd[rr++] = r;
+  }
+  nsmi_tensor* Q = tensor_new(ctx, NSMI_F64, qr, qd);
+  nsmi_tensor* R = tensor_new(ctx, NSMI_F64, rr, rd);
+  try {
+    if (!rows_trailing) {
+      // row-major M (m x nc) = col-major M^T (nc x m); LQ of M^T: M^T = L Qt -> M = Qt^T L^T
+      check_rocsolver(rocsolver_dgelqf(ctx->blas, rocblas_int(nc), rocblas_int(m), a.as<double>(), rocblas_int(nc),
+                                       tau.as<double>()),
+                      "rocsolver_dgelqf");
+      // L (col-major nc x r, lower) read row-major: first r rows of the m x nc buffer, keep j >= i
+      hipLaunchKernelGGL(k_diag_sign, dim3(blocks(r)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r,
+                         sg.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_extract_tri, dim3(blocks(r * nc)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r, nc,
+                         sg.as<double>(), static_cast<double*>(R->data), 0);
+      NSMI_LAUNCH_CHECK();
+      // Qt (r x m col-major) read row-major = Q (m x r)
+      check_rocsolver(rocsolver_dorglq(ctx->blas, rocblas_int(r), rocblas_int(m), rocblas_int(r), a.as<double>(),
+                                       rocblas_int(nc), tau.as<double>()),
+                      "rocsolver_dorglq");
+      // dorglq leaves Qt in the leading r rows (ld = nc) of the col-major buffer;
+      // as row-major it is m rows of stride nc, first r columns
+      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(r * m)), dim3(256), 0, ctx->stream, a.as<double>(), nc,
+                         (const int*)nullptr, int(r), m, sg.as<double>(), (const double*)nullptr,
+                         static_cast<double*>(Q->data));
+      NSMI_LAUNCH_CHECK();
+    } else {
+      // X = M^T (nc x m) is col-major M (m x nc) read as... use QR of col-major M^T? The row-major
+      // buffer (m x nc) is col-major (nc x m) = M^T = X directly: QR of X.
+      check_rocsolver(rocsolver_dgeqrf(ctx->blas, rocblas_int(nc), rocblas_int(m), a.as<double>(), rocblas_int(nc),
+                                       tau.as<double>()),
+                      "rocsolver_dgeqrf");
+      // R (col-major r x m, upper): element (i, j) at a[i + j*nc] = row-major (m x nc)[j][i];
+      // we need R^T (m x r row-major): R^T[j][i] = a[j*nc + i] for i <= j
+      hipLaunchKernelGGL(k_diag_sign, dim3(blocks(r)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r,
+                         sg.as<double>());
+      NSMI_LAUNCH_CHECK();
+      // rows j of the row-major buffer, entries i <= min(j, r-1); write R^T with sign on column i
+      hipLaunchKernelGGL(k_extract_tri_t, dim3(blocks(r * m)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r, m,
+                         sg.as<double>(), static_cast<double*>(R->data));
+      NSMI_LAUNCH_CHECK();
+      check_rocsolver(rocsolver_dorgqr(ctx->blas, rocblas_int(nc), rocblas_int(r), rocblas_int(r), a.as<double>(),
+                                       rocblas_int(nc), tau.as<double>()),
+                      "rocsolver_dorgqr");
+      // Q (col-major nc x r) read row-major = Q^T (r x nc): the site tensor; rows get the signs
+      NSMI_HIP(hipMemcpyAsync(Q->data, a.p, sizeof(double) * r * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_scale_rows, dim3(blocks(r * nc)), dim3(256), 0, ctx->stream, static_cast<double*>(Q->data),
+                         r, nc, sg.as<double>());
+      NSMI_LAUNCH_CHECK();
+    }
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    tensor_delete(ctx, Q);
+    tensor_delete(ctx, R);
+    throw;
+  }
+  *Qout = Q;
+  *Rout = R;
+  API_END(ctx)
+}
+
+}  // extern "C"

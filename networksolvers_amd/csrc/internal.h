@@ -1,0 +1,189 @@
+// Internal declarations of libnsmi.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nsmi.h"
+
+namespace nsmi {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+inline void require(bool ok, const std::string& msg) {
+  if (!ok) fail(NSMI_ERR_INVALID, msg);
+}
+
+#define NSMI_HIP(call)                                                                        \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      ::nsmi::fail(NSMI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));              \
+  } while (0)
+
+#define NSMI_LAUNCH_CHECK() NSMI_HIP(hipGetLastError())
+
+// Size-class caching allocator.  All work of a context runs on one stream, so a
+// block returned to the cache may be handed out again immediately: later work
+// using it is ordered after earlier work on the same stream.
+class Pool {
+ public:
+  explicit Pool(size_t limit) : limit_(limit) {}
+  ~Pool();
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();
+  size_t in_use() const { return in_use_; }
+  size_t cached() const { return cached_; }
+  size_t peak() const { return peak_; }
+
+ private:
+  static size_t round(size_t b);
+  size_t limit_;
+  size_t in_use_ = 0, cached_ = 0, peak_ = 0;
+  std::map<size_t, std::vector<void*>> free_;
+  std::map<void*, size_t> live_;
+};
+
+struct ProfRec {
+  hipEvent_t a, b;
+  int cls;
+  double flops, bytes;
+};
+
+}  // namespace nsmi
+
+struct nsmi_tensor {
+  int dtype;
+  int rank;
+  int64_t dims[8];
+  int64_t numel;
+  void* data;
+  size_t bytes;
+};
+
+struct nsmi_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  nsmi::Pool* pool = nullptr;
+  std::string err;
+  // reduction scratch
+  double* d_partials = nullptr;  // kPartials doubles
+  double* d_scal = nullptr;      // 64 result slots
+  double* h_scal = nullptr;      // pinned mirror of d_scal
+  int* d_info = nullptr;
+  // profiling
+  bool prof_on = false;
+  std::vector<nsmi::ProfRec> recs;
+  std::vector<hipEvent_t> spare_events;
+  nsmi_profile_stats stats[NSMI_K_COUNT] = {};
+};
+
+namespace nsmi {
+
+constexpr int kPartials = 1024;  // blocks of the first reduction pass
+
+inline size_t dtype_size(int dt) {
+  switch (dt) {
+    case NSMI_F64: return 8;
+    case NSMI_C128: return 16;
+    default: fail(NSMI_ERR_INVALID, "unknown dtype " + std::to_string(dt));
+  }
+}
+
+nsmi_tensor* tensor_new(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims);
+void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t);
+void* scratch(nsmi_ctx* ctx, size_t bytes);        // pool block, caller releases
+void scratch_release(nsmi_ctx* ctx, void* p);
+
+// RAII holder for pool scratch
+struct Scratch {
+  nsmi_ctx* ctx;
+  void* p;
+  Scratch(nsmi_ctx* c, size_t bytes) : ctx(c), p(scratch(c, bytes)) {}
+  ~Scratch() { if (p) scratch_release(ctx, p); }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+};
+
+// profiling brackets (no-ops when profiling is off)
+struct ProfScope {
+  nsmi_ctx* ctx;
+  int cls;
+  double flops, bytes;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(nsmi_ctx* c, int k, double f, double by);
+  ~ProfScope();
+};
+
+// ---- dense kernels (gemm.hip) -------------------------------------------
+// Row-major C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]; op(A) is M x K
+// (A stored K x M when transA), op(B) is K x N (B stored N x K when transB).
+struct GemmDesc {
+  int64_t M, N, K;
+  int64_t lda, ldb, ldc;
+  int64_t sA, sB, sC;  // batch strides in elements
+  int64_t batch;
+  bool transA, transB;
+  bool tiles_ok() const;
+};
+void gemm_f64(nsmi_ctx* ctx, const GemmDesc& g, double alpha, const double* A, const double* B,
+              double beta, double* C, int role);
+// FP32 MFMA on float operands, FP32 result
+void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, float* C, int role);
+// BF16 MFMA; operands pre-split into hi (and lo) planes; terms = 1 or 3; FP32 result
+void gemm_bf16(nsmi_ctx* ctx, const GemmDesc& g, const uint16_t* Ahi, const uint16_t* Alo,
+               const uint16_t* Bhi, const uint16_t* Blo, float* C, int terms, int role);
+
+// roles select distinct kernel instantiations so each call site is a separate
+// kernel name in profiles
+enum GemmRole { kRoleHeff1 = 0, kRoleHeff2 = 1, kRoleEnv = 2, kRoleTheta = 3, kRoleFact = 4 };
+
+// ---- elementwise / reduction kernels (vec.hip) -----------------------------
+void vec_scale(nsmi_ctx* ctx, double* x, int64_t n, double a);
+void vec_axpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double a);
+void vec_copy(nsmi_ctx* ctx, double* y, const double* x, int64_t n);
+// y = a*x  (separate output)
+void vec_scaled_copy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double a);
+// slot <- sum x*y (device), returns after enqueue
+void vec_dot_dev(nsmi_ctx* ctx, const double* x, const double* y, int64_t n, int slot);
+// w += c1*v1 + c2*v2 then slot <- |w|^2  (v2 may be null)
+void vec_axpy2_norm2_dev(nsmi_ctx* ctx, double* w, const double* v1, double c1, const double* v2,
+                         double c2, int64_t n, int slot);
+// out = sum_j c[j] * V[j]
+void vec_lincomb(nsmi_ctx* ctx, double* out, const double* const* V, const double* c, int k, int64_t n);
+// read result slots [0, count) to host (synchronises)
+void read_slots(nsmi_ctx* ctx, int count, double* out);
+void vec_to_f32(nsmi_ctx* ctx, float* y, const double* x, int64_t n);
+void vec_f32_to_f64(nsmi_ctx* ctx, double* y, const float* x, int64_t n);
+void vec_split_bf16(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const double* x, int64_t n);
+void vec_split_bf16_f32(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const float* x, int64_t n);
+void vec_check_finite(nsmi_ctx* ctx, const double* x, int64_t n, int slot);
+
+// ---- MPO mixing (region.hip) --------------------------------------------
+// Y[o][p][i] = sum_q Mx[p][q] X[o][q][i]
+void mix_f64(nsmi_ctx* ctx, const double* X, double* Y, const double* Mx, int64_t O, int P, int Q,
+             int64_t I, int cls);
+void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t O, int P, int Q,
+             int64_t I, int cls);
+// mixing whose output is written as bf16 hi/lo planes (feeds a BF16 GEMM)
+void mix_f32_to_bf16(nsmi_ctx* ctx, const float* X, uint16_t* Yhi, uint16_t* Ylo, const double* Mx,
+                     int64_t O, int P, int Q, int64_t I, int cls);
+
+void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jacobi, ascending
+
+}  // namespace nsmi

@@ -1,0 +1,582 @@
+// Region-step contractions: two-site / one-site effective Hamiltonian apply,
+// environment updates, theta formation and the Lanczos eigensolver.
+//
+// H_eff apply (operator_map.jl:3-10, order ((L.theta).W1W2).R):
+//   GEMM1  T1[(a' w), (s1 s2 b)] = L[(a' w), a] . theta[a, (s1 s2 b)]          (NN)
+//   MIX    T3[a', (s1' s2' v), b] = sum_{w s1 s2} W12[(s1' s2' v), (w s1 s2)] T1 (per a', b)
+//   GEMM2  y[(a' s1' s2'), b'] = T3[(a' s1' s2'), (v b)] . R[b', (v b)]^T        (NT)
+// W12 = W1 . W2 over the shared MPO bond is formed once per operator on the host.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "internal.h"
+
+using namespace nsmi;
+
+struct nsmi_heff {
+  const nsmi_tensor* L = nullptr;
+  const nsmi_tensor* R = nullptr;
+  int precision = NSMI_PREC_FP64;
+  int nsite = 2;
+  int64_t cl = 0, cr = 0, D0 = 0, D2 = 0, dphys = 0;  // dphys = d1*d2 (2-site) or d
+  int P = 0, Q = 0;
+  double* Mx = nullptr;  // device P x Q mixing matrix
+  void* T1 = nullptr;    // GEMM1 output
+  void* T3 = nullptr;    // mixing output
+  float* L32 = nullptr;
+  float* R32 = nullptr;
+  uint16_t *Lhi = nullptr, *Llo = nullptr, *Rhi = nullptr, *Rlo = nullptr;
+  double flops_gemm1 = 0, flops_gemm2 = 0, flops_mix = 0;
+  int64_t n = 0;  // elements of theta
+};
+
+namespace nsmi {
+
+namespace {
+
+constexpr int kMixThreads = 256;
+
+template <typename TI, typename TO, int QMAX>
+__global__ __launch_bounds__(kMixThreads) void k_mix(const TI* __restrict__ X, TO* __restrict__ Y,
+                                                     const double* __restrict__ Mx, int64_t O, int P, int Q,
+                                                     int64_t I) {
+  __shared__ double sM[64 * 64];
+  for (int t = threadIdx.x; t < P * Q; t += blockDim.x) sM[t] = Mx[t];
+  __syncthreads();
+  const int64_t i = int64_t(blockIdx.x) * kMixThreads + threadIdx.x;
+  if (i >= I) return;
+  for (int64_t o = blockIdx.y; o < O; o += gridDim.y) {
+    const TI* xo = X + o * Q * I + i;
+    TO* yo = Y + o * int64_t(P) * I + i;
+    double x[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) x[q] = (q < Q) ? double(xo[int64_t(q) * I]) : 0.0;
+    for (int p = 0; p < P; ++p) {
+      const double* m = sM + p * Q;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < QMAX; ++q)
+        if (q < Q) acc += m[q] * x[q];
+      yo[int64_t(p) * I] = TO(acc);
+    }
+  }
+}
+
+template <typename TI, typename TO>
+void mix_launch(nsmi_ctx* ctx, const TI* X, TO* Y, const double* Mx, int64_t O, int P, int Q, int64_t I,
+                int cls) {
+  require(P >= 1 && Q >= 1 && P <= 64 && Q <= 64, "mix: P, Q must be in [1, 64]");
+  if (O == 0 || I == 0) return;
+  ProfScope ps(ctx, cls, 2.0 * double(O) * double(I) * P * Q,
+               double(O) * double(I) * (P * sizeof(TO) + Q * sizeof(TI)));
+  dim3 grid(unsigned((I + kMixThreads - 1) / kMixThreads), unsigned(std::min<int64_t>(O, 65535)));
+  if (Q <= 8)
+    hipLaunchKernelGGL((k_mix<TI, TO, 8>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+  else if (Q <= 16)
+    hipLaunchKernelGGL((k_mix<TI, TO, 16>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+  else if (Q <= 32)
+    hipLaunchKernelGGL((k_mix<TI, TO, 32>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+  else
+    hipLaunchKernelGGL((k_mix<TI, TO, 64>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+  NSMI_LAUNCH_CHECK();
+}
+
+GemmDesc desc(int64_t M, int64_t N, int64_t K, bool ta, bool tb) {
+  GemmDesc g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.transA = ta;
+  g.transB = tb;
+  g.lda = ta ? M : K;
+  g.ldb = tb ? K : N;
+  g.ldc = N;
+  g.batch = 1;
+  return g;
+}
+
+const double* dptr(const nsmi_tensor* t) { return static_cast<const double*>(t->data); }
+double* dptr(nsmi_tensor* t) { return static_cast<double*>(t->data); }
+
+void check_f64(const nsmi_tensor* t, const char* what) {
+  require(t != nullptr, std::string(what) + " is null");
+  if (t->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, std::string(what) + ": only F64 tensors are served");
+}
+
+size_t prec_elt(int prec) { return prec == NSMI_PREC_FP64 ? 8 : 4; }
+
+// Mx[(s1' s2' v), (w s1 s2)] = sum_u W1[w,u,s1',s1] W2[u,v,s2',s2]
+__global__ void k_build_w12(const double* __restrict__ W1, const double* __restrict__ W2, double* __restrict__ Mx,
+                            int D0, int Dm, int D2, int d1, int d2) {
+  const int P = d1 * d2 * D2, Q = D0 * d1 * d2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * Q) return;
+  const int p = t / Q, q = t % Q;
+  const int v = p % D2, a2 = (p / D2) % d2, a1 = p / (D2 * d2);
+  const int b2 = q % d2, b1 = (q / d2) % d1, w = q / (d1 * d2);
+  double acc = 0.0;
+  for (int u = 0; u < Dm; ++u)
+    acc += W1[((w * Dm + u) * d1 + a1) * d1 + b1] * W2[((u * D2 + v) * d2 + a2) * d2 + b2];
+  Mx[t] = acc;
+}
+
+// mode 0: Mx[(s' v), (w s)] = W[w, v, s', s]     (H_eff 1-site, left environment)
+// mode 1: Mx[(w s), (s' v)] = W[w, v, s', s]     (right environment)
+__global__ void k_build_w1(const double* __restrict__ W, double* __restrict__ Mx, int Dl, int Dr, int d, int mode) {
+  const int n = Dl * Dr * d * d;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int s = t % d, sp = (t / d) % d, v = (t / (d * d)) % Dr, w = t / (d * d * Dr);
+  const double x = W[t];
+  if (mode == 0) {
+    const int Q = Dl * d;
+    Mx[(sp * Dr + v) * Q + (w * d + s)] = x;
+  } else {
+    const int Q = d * Dr;
+    Mx[(w * d + s) * Q + (sp * Dr + v)] = x;
+  }
+}
+
+void build_w1(nsmi_ctx* ctx, const double* W, double* Mx, int Dl, int Dr, int d, int mode) {
+  const int n = Dl * Dr * d * d;
+  hipLaunchKernelGGL(k_build_w1, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, W, Mx, Dl, Dr, d, mode);
+  NSMI_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void mix_f64(nsmi_ctx* ctx, const double* X, double* Y, const double* Mx, int64_t O, int P, int Q, int64_t I,
+             int cls) {
+  mix_launch<double, double>(ctx, X, Y, Mx, O, P, Q, I, cls);
+}
+void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t O, int P, int Q, int64_t I,
+             int cls) {
+  mix_launch<float, float>(ctx, X, Y, Mx, O, P, Q, I, cls);
+}
+
+// Cyclic Jacobi eigensolver for small dense symmetric matrices (host), used for
+// the Krylov projected problems; eigenvalues ascending, eigenvectors as columns
+// of V (row-major n x n).
+void small_sym_eigh(int n, const double* Ain, double* w, double* V) {
+  std::vector<double> A(Ain, Ain + size_t(n) * n);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) V[i * n + j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-34 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        const double app = A[p * n + p], aqq = A[q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // A <- A J (columns p, q)
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // A <- J^T A (rows p, q)
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {  // V <- V J
+          const double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::vector<double> d(n);
+  for (int i = 0; i < n; ++i) d[i] = A[i * n + i];
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+  std::vector<double> Vs(size_t(n) * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = d[idx[j]];
+    for (int i = 0; i < n; ++i) Vs[i * n + j] = V[i * n + idx[j]];
+  }
+  std::memcpy(V, Vs.data(), sizeof(double) * n * n);
+}
+
+}  // namespace nsmi
+
+// ------------------------------------------------------------------ H_eff
+static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y, int prec) {
+  const int64_t M1 = op->cl * op->D0, K1 = op->cl, N1 = op->dphys * op->cr;
+  const int64_t M2 = op->cl * op->dphys, K2 = op->D2 * op->cr, N2 = op->cr;
+  GemmDesc g1 = desc(M1, N1, K1, false, false);
+  GemmDesc g2 = desc(M2, N2, K2, false, true);
+  if (prec == NSMI_PREC_FP64) {
+    double* T1 = static_cast<double*>(op->T1);
+    double* T3 = static_cast<double*>(op->T3);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 8.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      gemm_f64(ctx, g1, 1.0, dptr(op->L), x, 0.0, T1, kRoleHeff1);
+    }
+    mix_f64(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, op->cr, NSMI_K_HEFF_MIX);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 8.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      gemm_f64(ctx, g2, 1.0, T3, dptr(op->R), 0.0, y, kRoleHeff2);
+    }
+    return;
+  }
+  if (prec == NSMI_PREC_FP32) {
+    float* T1 = static_cast<float*>(op->T1);
+    float* T3 = static_cast<float*>(op->T3);
+    Scratch xs(ctx, sizeof(float) * op->n), ys(ctx, sizeof(float) * op->n);
+    vec_to_f32(ctx, xs.as<float>(), x, op->n);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 4.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      gemm_f32(ctx, g1, op->L32, xs.as<float>(), T1, kRoleHeff1);
+    }
+    mix_f32(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, op->cr, NSMI_K_HEFF_MIX);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 4.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      gemm_f32(ctx, g2, T3, op->R32, ys.as<float>(), kRoleHeff2);
+    }
+    vec_f32_to_f64(ctx, y, ys.as<float>(), op->n);
+    return;
+  }
+  fail(NSMI_ERR_UNSUPPORTED, "H_eff precision not served by this build");
+}
+
+extern "C" {
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                             \
+  }                                                               \
+  catch (const nsmi::Error& e) {                                  \
+    if (ctxp) (ctxp)->err = e.what();                             \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what(); \
+    return NSMI_ERR_INVALID;                                      \
+  }                                                               \
+  return NSMI_OK;
+
+int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1, const nsmi_tensor* W2,
+                     const nsmi_tensor* R, int precision, nsmi_heff** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  nsmi_heff* op = nullptr;
+  API_BEGIN
+  *out = nullptr;
+  check_f64(L, "L");
+  check_f64(R, "R");
+  check_f64(W1, "W1");
+  require(L->rank == 3 && R->rank == 3 && W1->rank == 4, "heff: L, R must be rank 3 and W rank 4");
+  require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32 || precision == NSMI_PREC_BF16X3 ||
+              precision == NSMI_PREC_BF16,
+          "heff: unknown precision");
+  if (precision == NSMI_PREC_BF16X3 || precision == NSMI_PREC_BF16)
+    fail(NSMI_ERR_UNSUPPORTED, "heff: BF16 precisions not served by this build");
+  op = new nsmi_heff();
+  op->L = L;
+  op->R = R;
+  op->precision = precision;
+  op->cl = L->dims[0];
+  op->cr = R->dims[0];
+  op->D0 = L->dims[1];
+  require(L->dims[2] == op->cl && R->dims[2] == op->cr, "heff: environments must be square in the links");
+  require(W1->dims[0] == op->D0 && W1->dims[2] == W1->dims[3], "heff: W1 does not match L");
+  const int64_t d1 = W1->dims[2];
+  if (W2) {
+    check_f64(W2, "W2");
+    require(W2->rank == 4 && W2->dims[0] == W1->dims[1] && W2->dims[2] == W2->dims[3], "heff: W2 does not match W1");
+    const int64_t Dm = W1->dims[1], d2 = W2->dims[2];
+    op->nsite = 2;
+    op->D2 = W2->dims[1];
+    op->dphys = d1 * d2;
+    require(R->dims[1] == op->D2, "heff: R does not match W2");
+    op->Q = int(op->D0 * d1 * d2);
+    op->P = int(d1 * d2 * op->D2);
+    require(op->P <= 64 && op->Q <= 64, "heff: D*d^2 above 64 is not served");
+    op->Mx = static_cast<double*>(scratch(ctx, sizeof(double) * op->P * op->Q));
+    const int np = op->P * op->Q;
+    hipLaunchKernelGGL(k_build_w12, dim3((np + 255) / 256), dim3(256), 0, ctx->stream, dptr(W1), dptr(W2), op->Mx,
+                       int(op->D0), int(Dm), int(op->D2), int(d1), int(d2));
+    NSMI_LAUNCH_CHECK();
+  } else {
+    op->nsite = 1;
+    op->D2 = W1->dims[1];
+    op->dphys = d1;
+    require(R->dims[1] == op->D2, "heff: R does not match W");
+    op->Q = int(op->D0 * d1);
+    op->P = int(d1 * op->D2);
+    require(op->P <= 64 && op->Q <= 64, "heff: D*d above 64 is not served");
+    op->Mx = static_cast<double*>(scratch(ctx, sizeof(double) * op->P * op->Q));
+    build_w1(ctx, dptr(W1), op->Mx, int(op->D0), int(op->D2), int(d1), 0);
+  }
+  op->n = op->cl * op->dphys * op->cr;
+  const size_t elt = prec_elt(precision);
+  op->T1 = scratch(ctx, elt * size_t(op->cl * op->D0 * op->dphys * op->cr));
+  op->T3 = scratch(ctx, elt * size_t(op->cl * op->D2 * op->dphys * op->cr));
+  op->flops_gemm1 = 2.0 * double(op->cl * op->D0) * double(op->dphys * op->cr) * double(op->cl);
+  op->flops_gemm2 = 2.0 * double(op->cl * op->dphys) * double(op->cr) * double(op->D2 * op->cr);
+  op->flops_mix = 2.0 * double(op->cl) * double(op->cr) * op->P * op->Q;
+  if (precision == NSMI_PREC_FP32) {
+    op->L32 = static_cast<float*>(scratch(ctx, sizeof(float) * L->numel));
+    op->R32 = static_cast<float*>(scratch(ctx, sizeof(float) * R->numel));
+    vec_to_f32(ctx, op->L32, dptr(L), L->numel);
+    vec_to_f32(ctx, op->R32, dptr(R), R->numel);
+  }
+  *out = op;
+  op = nullptr;
+  API_END(ctx)
+}
+
+int nsmi_heff_destroy(nsmi_ctx* ctx, nsmi_heff* op) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  if (!op) return NSMI_OK;
+  API_BEGIN
+  scratch_release(ctx, op->Mx);
+  scratch_release(ctx, op->T1);
+  scratch_release(ctx, op->T3);
+  scratch_release(ctx, op->L32);
+  scratch_release(ctx, op->R32);
+  scratch_release(ctx, op->Lhi);
+  scratch_release(ctx, op->Llo);
+  scratch_release(ctx, op->Rhi);
+  scratch_release(ctx, op->Rlo);
+  delete op;
+  API_END(ctx)
+}
+
+int nsmi_heff_apply(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y) {
+  if (!ctx || !op) return NSMI_ERR_INVALID;
+  API_BEGIN
+  check_f64(x, "x");
+  check_f64(y, "y");
+  require(x->numel == op->n && y->numel == op->n, "heff_apply: x/y size does not match the operator");
+  require(x->data != y->data, "heff_apply: x and y must not alias");
+  heff_apply_impl(ctx, op, dptr(x), dptr(y), op->precision);
+  API_END(ctx)
+}
+
+int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, double* e) {
+  if (!ctx || !op || !e) return NSMI_ERR_INVALID;
+  API_BEGIN
+  check_f64(x, "x");
+  require(x->numel == op->n, "rayleigh: size mismatch");
+  // an FP64 apply needs FP64-sized workspace
+  void* T1 = op->T1;
+  void* T3 = op->T3;
+  Scratch t1(ctx, op->precision == NSMI_PREC_FP64 ? 8 : 8 * size_t(op->cl * op->D0 * op->dphys * op->cr));
+  Scratch t3(ctx, op->precision == NSMI_PREC_FP64 ? 8 : 8 * size_t(op->cl * op->D2 * op->dphys * op->cr));
+  if (op->precision != NSMI_PREC_FP64) {
+    op->T1 = t1.p;
+    op->T3 = t3.p;
+  }
+  Scratch ys(ctx, 8 * size_t(op->n));
+  try {
+    heff_apply_impl(ctx, op, dptr(x), ys.as<double>(), NSMI_PREC_FP64);
+  } catch (...) {
+    op->T1 = T1;
+    op->T3 = T3;
+    throw;
+  }
+  op->T1 = T1;
+  op->T3 = T3;
+  vec_dot_dev(ctx, dptr(x), ys.as<double>(), op->n, 0);
+  vec_dot_dev(ctx, dptr(x), dptr(x), op->n, 1);
+  double r[2];
+  read_slots(ctx, 2, r);
+  require(r[1] > 0.0, "rayleigh: zero vector");
+  *e = r[0] / r[1];
+  API_END(ctx)
+}
+
+// ----------------------------------------------------------- environments
+int nsmi_env_update(nsmi_ctx* ctx, const nsmi_tensor* E, const nsmi_tensor* A, const nsmi_tensor* W, int side,
+                    int precision, nsmi_tensor** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *out = nullptr;
+  check_f64(E, "E");
+  check_f64(A, "A");
+  check_f64(W, "W");
+  require(E->rank == 3 && A->rank == 3 && W->rank == 4, "env_update: ranks must be E:3, A:3, W:4");
+  require(side == 0 || side == 1, "env_update: side must be 0 (left) or 1 (right)");
+  require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32, "env_update: precision not served");
+  const int64_t ca = A->dims[0], d = A->dims[1], cb = A->dims[2];
+  require(W->dims[2] == d && W->dims[3] == d, "env_update: W physical dims do not match A");
+  const int64_t Dl = W->dims[0], Dr = W->dims[1];
+  nsmi_tensor* res = nullptr;
+  if (side == 0) {
+    require(E->dims[0] == ca && E->dims[2] == ca && E->dims[1] == Dl, "env_update(left): E does not match A/W");
+    const int Q = int(Dl * d), P = int(d * Dr);
+    require(P <= 64 && Q <= 64, "env_update: D*d above 64 is not served");
+    Scratch dmx(ctx, size_t(P) * Q * 8);  // Mx[(s' w'), (w s)] = W[w, w', s', s]
+    build_w1(ctx, dptr(W), dmx.as<double>(), int(Dl), int(Dr), int(d), 0);
+    const int64_t dims[3] = {cb, Dr, cb};
+    res = tensor_new(ctx, NSMI_F64, 3, dims);
+    Scratch X(ctx, 8 * size_t(ca * Dl * d * cb)), Y(ctx, 8 * size_t(ca * d * Dr * cb));
+    const double fl = 2.0 * ca * Dl * ca * d * cb + 2.0 * cb * Dr * cb * ca * d;
+    ProfScope ps(ctx, NSMI_K_ENV, fl + 2.0 * ca * cb * P * Q, 8.0 * (E->numel + 2 * A->numel + res->numel));
+    gemm_f64(ctx, desc(ca * Dl, d * cb, ca, false, false), 1.0, dptr(E), dptr(A), 0.0, X.as<double>(), kRoleEnv);
+    mix_f64(ctx, X.as<double>(), Y.as<double>(), dmx.as<double>(), ca, P, Q, cb, -1);
+    gemm_f64(ctx, desc(cb, Dr * cb, ca * d, true, false), 1.0, dptr(A), Y.as<double>(), 0.0, dptr(res), kRoleEnv);
+  } else {
+    require(E->dims[0] == cb && E->dims[2] == cb && E->dims[1] == Dr, "env_update(right): E does not match A/W");
+    const int Q = int(d * Dr), P = int(Dl * d);
+    require(P <= 64 && Q <= 64, "env_update: D*d above 64 is not served");
+    Scratch dmx(ctx, size_t(P) * Q * 8);  // Mx[(w s), (s' v)] = W[w, v, s', s]
+    build_w1(ctx, dptr(W), dmx.as<double>(), int(Dl), int(Dr), int(d), 1);
+    const int64_t dims[3] = {ca, Dl, ca};
+    res = tensor_new(ctx, NSMI_F64, 3, dims);
+    Scratch X(ctx, 8 * size_t(ca * d * Dr * cb)), Y(ctx, 8 * size_t(ca * Dl * d * cb));
+    const double fl = 2.0 * ca * d * cb * Dr * cb + 2.0 * ca * Dl * ca * d * cb;
+    ProfScope ps(ctx, NSMI_K_ENV, fl + 2.0 * ca * cb * P * Q, 8.0 * (E->numel + 2 * A->numel + res->numel));
+    gemm_f64(ctx, desc(ca * d, Dr * cb, cb, false, false), 1.0, dptr(A), dptr(E), 0.0, X.as<double>(), kRoleEnv);
+    mix_f64(ctx, X.as<double>(), Y.as<double>(), dmx.as<double>(), ca, P, Q, cb, -1);
+    gemm_f64(ctx, desc(ca * Dl, ca, d * cb, false, true), 1.0, Y.as<double>(), dptr(A), 0.0, dptr(res), kRoleEnv);
+  }
+  *out = res;
+  API_END(ctx)
+}
+
+int nsmi_theta(nsmi_ctx* ctx, const nsmi_tensor* A, const nsmi_tensor* B, nsmi_tensor** theta) {
+  if (!ctx || !theta) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *theta = nullptr;
+  check_f64(A, "A");
+  check_f64(B, "B");
+  require(A->rank == 3 && B->rank == 3 && A->dims[2] == B->dims[0], "theta: A[l,s,m] and B[m,s,r] required");
+  const int64_t cl = A->dims[0], d1 = A->dims[1], cm = A->dims[2], d2 = B->dims[1], cr = B->dims[2];
+  const int64_t dims[4] = {cl, d1, d2, cr};
+  nsmi_tensor* t = tensor_new(ctx, NSMI_F64, 4, dims);
+  ProfScope ps(ctx, NSMI_K_THETA, 2.0 * cl * d1 * cm * d2 * cr, 8.0 * (A->numel + B->numel + t->numel));
+  gemm_f64(ctx, desc(cl * d1, d2 * cr, cm, false, false), 1.0, dptr(A), dptr(B), 0.0, dptr(t), kRoleTheta);
+  *theta = t;
+  API_END(ctx)
+}
+
+// ------------------------------------------------------------------ Lanczos
+// Restates KrylovKit eigsolve(f, x0, 1, :SR; ishermitian=true, krylovdim, maxiter=1,
+// tol, eager=false) with the ModifiedGramSchmidtIR(eta) orthogonaliser
+// (local_solvers/eigsolve.jl:3-29).  Scalars come back to the host once per
+// reduction; the vectors never leave HBM.
+int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_lanczos_opts* opts, double* eigval,
+                     nsmi_solver_info* info) {
+  if (!ctx || !op || !x || !eigval) return NSMI_ERR_INVALID;
+  API_BEGIN
+  check_f64(x, "x");
+  require(x->numel == op->n, "lanczos: x does not match the operator");
+  int K = opts ? opts->krylovdim : 3;
+  const int maxiter = opts ? opts->maxiter : 1;
+  const double tol = opts ? opts->tol : 1e-14;
+  const double eta = opts ? opts->eta : 1.0 / std::sqrt(2.0);
+  require(K >= 1 && K <= 32, "lanczos: krylovdim must be in [1, 32]");
+  if (maxiter != 1) fail(NSMI_ERR_UNSUPPORTED, "lanczos: only maxiter = 1 is served");
+  const int64_t n = op->n;
+  const double epsv = 2.220446049250313e-16;
+  Scratch basis(ctx, sizeof(double) * size_t(n) * (K + 1));
+  double* V = basis.as<double>();
+  double* r = V + size_t(n) * K;  // residual / work vector
+  double* xd = dptr(x);
+
+  double s[2];
+  vec_dot_dev(ctx, xd, xd, n, 0);
+  read_slots(ctx, 1, s);
+  const double beta0 = std::sqrt(s[0]);
+  require(beta0 > 0.0, "lanczos: initial vector should not have norm zero");
+  // Ax0 -> r
+  heff_apply_impl(ctx, op, xd, r, op->precision);
+  int numops = 1;
+  vec_dot_dev(ctx, xd, r, n, 0);
+  read_slots(ctx, 1, s);
+  double alpha = s[0] / (beta0 * beta0);
+  vec_scaled_copy(ctx, V, xd, n, 1.0 / beta0);   // v1 = x0 / beta0
+  vec_scale(ctx, r, n, 1.0 / beta0);             // r = Ax0 / beta0
+  vec_dot_dev(ctx, r, r, n, 0);
+  read_slots(ctx, 1, s);
+  double beta_old = std::sqrt(s[0]);
+  vec_axpy2_norm2_dev(ctx, r, V, -alpha, nullptr, 0.0, n, 0);
+  read_slots(ctx, 1, s);
+  double beta = std::sqrt(s[0]);
+  while (epsv < beta && beta < eta * beta_old) {
+    beta_old = beta;
+    vec_dot_dev(ctx, V, r, n, 0);
+    read_slots(ctx, 1, s);
+    alpha += s[0];
+    vec_axpy2_norm2_dev(ctx, r, V, -s[0], nullptr, 0.0, n, 0);
+    read_slots(ctx, 1, s);
+    beta = std::sqrt(s[0]);
+  }
+  std::vector<double> alphas{alpha}, betas;
+  int k = 1;
+  while (!(k == K || beta <= tol)) {
+    double* vk = V + size_t(n) * k;
+    double* vprev = V + size_t(n) * (k - 1);
+    vec_scaled_copy(ctx, vk, r, n, 1.0 / beta);
+    betas.push_back(beta);
+    heff_apply_impl(ctx, op, vk, r, op->precision);  // w
+    ++numops;
+    // w -= beta * v_{k-1}; a = <v_k, w>
+    vec_axpy(ctx, r, vprev, n, -beta);
+    vec_dot_dev(ctx, vk, r, n, 0);
+    read_slots(ctx, 1, s);
+    double a = s[0];
+    vec_axpy2_norm2_dev(ctx, r, vk, -a, nullptr, 0.0, n, 0);
+    read_slots(ctx, 1, s);
+    double b = std::sqrt(s[0]);
+    double nold = std::sqrt(b * b + a * a + beta * beta);
+    while (epsv < b && b < eta * nold) {
+      nold = b;
+      double c = 0.0;
+      for (int j = 0; j <= k; ++j) {  // modified Gram-Schmidt against the whole basis
+        const double* q = V + size_t(n) * j;
+        vec_dot_dev(ctx, q, r, n, 0);
+        read_slots(ctx, 1, s);
+        c = s[0];
+        vec_axpy(ctx, r, q, n, -c);
+      }
+      a += c;
+      vec_dot_dev(ctx, r, r, n, 0);
+      read_slots(ctx, 1, s);
+      b = std::sqrt(s[0]);
+    }
+    alphas.push_back(a);
+    beta = b;
+    ++k;
+  }
+  // projected tridiagonal problem
+  std::vector<double> T(size_t(k) * k, 0.0), w(k), Y(size_t(k) * k);
+  for (int i = 0; i < k; ++i) T[i * k + i] = alphas[i];
+  for (int i = 0; i + 1 < k; ++i) T[i * k + i + 1] = T[(i + 1) * k + i] = betas[i];
+  small_sym_eigh(k, T.data(), w.data(), Y.data());
+  std::vector<double> y(k);
+  double ymax = 0.0;
+  for (int i = 0; i < k; ++i) {
+    y[i] = Y[i * k + 0];
+    ymax = std::max(ymax, std::fabs(y[i]));
+  }
+  int i0 = 0;
+  while (i0 < k - 1 && !(std::fabs(y[i0]) > 1e-14 * ymax)) ++i0;
+  if (y[i0] < 0)
+    for (auto& v : y) v = -v;
+  std::vector<const double*> Vp(k);
+  for (int j = 0; j < k; ++j) Vp[j] = V + size_t(n) * j;
+  vec_lincomb(ctx, xd, Vp.data(), y.data(), k, n);
+  *eigval = w[0];
+  if (info) {
+    info->numops = numops;
+    info->krylov_dim = k;
+    info->normres = std::fabs(beta * y[k - 1]);
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  API_END(ctx)
+}
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+// Streaming FP64 vector kernels for the Krylov solvers: scale/axpy/copy, dot and
+// norm reductions, linear combinations, precision conversions.  All of them are
+// HBM-bound; they use grid-stride loops with 16-byte accesses where aligned.
+#include <cmath>
+
+#include "internal.h"
+
+namespace nsmi {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline int grid_for(int64_t n, int per_thread = 2) {
+  int64_t g = (n + int64_t(kThreads) * per_thread - 1) / (int64_t(kThreads) * per_thread);
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return int(g);
+}
+
+__global__ void k_scale(double* __restrict__ x, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void k_scaled_copy(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = a * x[i];
+}
+
+__device__ inline double block_sum(double v) {
+  __shared__ double red[kThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kThreads / 64; ++w) s += red[w];
+  __syncthreads();
+  return s;
+}
+
+__global__ void k_dot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                              double* __restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    acc += x[i] * y[i];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void k_axpy2_norm_partial(double* __restrict__ w, const double* __restrict__ v1, double c1,
+                                     const double* __restrict__ v2, double c2, int64_t n,
+                                     double* __restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double t = w[i];
+    if (v1) t += c1 * v1[i];
+    if (v2) t += c2 * v2[i];
+    w[i] = t;
+    acc += t * t;
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// final pass: one block sums the partials in a fixed order into slot
+__global__ void k_finish(const double* __restrict__ partials, int np, double* __restrict__ slot) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) *slot = s;
+}
+
+struct LinComb {
+  const double* v[32];
+  double c[32];
+};
+
+__global__ void k_lincomb(double* __restrict__ out, LinComb lc, int k, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < k; ++j) s += lc.c[j] * lc.v[j][i];
+    out[i] = s;
+  }
+}
+
+__global__ void k_to_f32(float* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = float(x[i]);
+}
+
+__global__ void k_f32_to_f64(double* __restrict__ y, const float* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] = double(x[i]);
+}
+
+// round-to-nearest-even FP32 -> BF16 (inputs here are finite)
+__device__ inline uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return uint16_t(u >> 16);
+}
+__device__ inline float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__global__ void k_split_bf16(uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, const double* __restrict__ x,
+                             int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double v = x[i];
+    uint16_t h = f32_to_bf16_rne(float(v));
+    hi[i] = h;
+    if (lo) lo[i] = f32_to_bf16_rne(float(v - double(bf16_to_f32(h))));
+  }
+}
+
+__global__ void k_split_bf16_f32(uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, const float* __restrict__ x,
+                                 int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    float v = x[i];
+    uint16_t h = f32_to_bf16_rne(v);
+    hi[i] = h;
+    if (lo) lo[i] = f32_to_bf16_rne(v - bf16_to_f32(h));
+  }
+}
+
+__global__ void k_nonfinite_partial(const double* __restrict__ x, int64_t n, double* __restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    acc += isfinite(x[i]) ? 0.0 : 1.0;
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+}  // namespace
+
+void vec_scale(nsmi_ctx* ctx, double* x, int64_t n, double a) {
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 16.0 * n);
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, x, n, a);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_axpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double a) {
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 24.0 * n);
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, a);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_copy(nsmi_ctx* ctx, double* y, const double* x, int64_t n) {
+  if (n <= 0) return;
+  NSMI_HIP(hipMemcpyAsync(y, x, size_t(n) * 8, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+void vec_scaled_copy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double a) {
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 16.0 * n);
+  hipLaunchKernelGGL(k_scaled_copy, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, a);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_dot_dev(nsmi_ctx* ctx, const double* x, const double* y, int64_t n, int slot) {
+  require(slot >= 0 && slot < 64, "slot out of range");
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, (x == y ? 8.0 : 16.0) * n);
+  int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, x, y, n, ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_partials, g, ctx->d_scal + slot);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_axpy2_norm2_dev(nsmi_ctx* ctx, double* w, const double* v1, double c1, const double* v2, double c2,
+                         int64_t n, int slot) {
+  require(slot >= 0 && slot < 64, "slot out of range");
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, (16.0 + (v1 ? 8.0 : 0.0) + (v2 ? 8.0 : 0.0)) * n);
+  int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_axpy2_norm_partial, dim3(g), dim3(kThreads), 0, ctx->stream, w, v1, c1, v2, c2, n,
+                     ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_partials, g, ctx->d_scal + slot);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_lincomb(nsmi_ctx* ctx, double* out, const double* const* V, const double* c, int k, int64_t n) {
+  require(k >= 1 && k <= 32, "lincomb: 1..32 vectors");
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 8.0 * (k + 1) * n);
+  LinComb lc{};
+  for (int j = 0; j < k; ++j) {
+    lc.v[j] = V[j];
+    lc.c[j] = c[j];
+  }
+  hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, out, lc, k, n);
+  NSMI_LAUNCH_CHECK();
+}
+
+void read_slots(nsmi_ctx* ctx, int count, double* out) {
+  require(count >= 1 && count <= 64, "read_slots count");
+  NSMI_HIP(hipMemcpyAsync(ctx->h_scal, ctx->d_scal, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < count; ++i) out[i] = ctx->h_scal[i];
+}
+
+void vec_to_f32(nsmi_ctx* ctx, float* y, const double* x, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_f32_to_f64(nsmi_ctx* ctx, double* y, const float* x, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_split_bf16(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const double* x, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_split_bf16, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, hi, lo, x, n);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_split_bf16_f32(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const float* x, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_split_bf16_f32, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, hi, lo, x, n);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_check_finite(nsmi_ctx* ctx, const double* x, int64_t n, int slot) {
+  int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_nonfinite_partial, dim3(g), dim3(kThreads), 0, ctx->stream, x, n, ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_partials, g, ctx->d_scal + slot);
+  NSMI_LAUNCH_CHECK();
+}
+
+}  // namespace nsmi

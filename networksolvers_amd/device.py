@@ -1,0 +1,170 @@
+"""Device context and tensor handles over libnsmi.so (HBM-resident, one HIP stream per context)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+_DT = {np.dtype(np.float64): _lib.F64, np.dtype(np.complex128): _lib.C128}
+_NP = {_lib.F64: np.float64, _lib.C128: np.complex128}
+
+
+class Context:
+    """One device context: a HIP stream, a caching device pool and a rocSOLVER handle.
+
+    ``pool_limit`` (bytes, 0 = unlimited) caps the pool; a request past it raises
+    NSMI_ERR_OOM, which the serving loop uses for admission by HBM footprint."""
+
+    def __init__(self, device: int = 0, pool_limit: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        code = self.lib.nsmi_ctx_create(int(device), C.c_size_t(int(pool_limit)), C.byref(h))
+        if code != 0:
+            raise _lib.NsmiError(code, f"nsmi_ctx_create(device={device}) failed")
+        self.h = h
+        self.device = device
+        self.closed = False
+
+    # ---------------------------------------------------------------- basics
+    def close(self):
+        if not self.closed:
+            self.lib.nsmi_ctx_destroy(self.h)
+            self.closed = True
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, code):
+        check(self.h, code)
+
+    def sync(self):
+        self.check(self.lib.nsmi_sync(self.h))
+
+    def pool_stats(self):
+        a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        self.check(self.lib.nsmi_pool_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"in_use": a.value, "cached": b.value, "peak": c.value}
+
+    def trim(self):
+        self.check(self.lib.nsmi_pool_trim(self.h))
+
+    # ------------------------------------------------------------- tensors
+    def upload(self, array) -> "DeviceTensor":
+        a = np.ascontiguousarray(array)
+        if a.dtype not in _DT:
+            a = a.astype(np.float64)
+        dims = (C.c_int64 * max(a.ndim, 1))(*a.shape)
+        h = C.c_void_p()
+        self.check(self.lib.nsmi_tensor_upload(self.h, _DT[a.dtype], a.ndim, dims,
+                                               a.ctypes.data_as(C.c_void_p), C.byref(h)))
+        return DeviceTensor(self, h)
+
+    def empty(self, shape, dtype=np.float64) -> "DeviceTensor":
+        dims = (C.c_int64 * max(len(shape), 1))(*shape)
+        h = C.c_void_p()
+        self.check(self.lib.nsmi_tensor_create(self.h, _DT[np.dtype(dtype)], len(shape), dims, C.byref(h)))
+        return DeviceTensor(self, h)
+
+    # ----------------------------------------------------------- profiling
+    def profile(self, on: bool = True):
+        self.check(self.lib.nsmi_profile_enable(self.h, int(on)))
+
+    def profile_reset(self):
+        self.check(self.lib.nsmi_profile_reset(self.h))
+
+    def profile_stats(self):
+        out = {}
+        for name, k in _lib.K_CLASSES.items():
+            s = _lib.ProfileStats()
+            self.check(self.lib.nsmi_profile_query(self.h, k, C.byref(s)))
+            out[name] = {"launches": s.launches, "total_ms": s.total_ms, "flops": s.flops, "bytes": s.bytes}
+        return out
+
+
+class DeviceTensor:
+    """Handle of a tensor in HBM owned by a Context (freed on garbage collection)."""
+
+    __slots__ = ("ctx", "h", "shape", "dtype", "__weakref__")
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+        self._refresh()
+
+    def _refresh(self):
+        dt, rk = C.c_int(), C.c_int()
+        dims = (C.c_int64 * 8)()
+        self.ctx.check(self.ctx.lib.nsmi_tensor_info(self.h, C.byref(dt), C.byref(rk), dims))
+        self.shape = tuple(int(dims[i]) for i in range(rk.value))
+        self.dtype = np.dtype(_NP[dt.value])
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape)) if self.shape else 1
+
+    @property
+    def ndim(self):
+        return len(self.shape)
+
+    def free(self):
+        if self.h is not None and not self.ctx.closed:
+            self.ctx.lib.nsmi_tensor_free(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.nsmi_tensor_download(self.ctx.h, self.h, out.ctypes.data_as(C.c_void_p),
+                                                         C.c_size_t(out.nbytes)))
+        return out
+
+    def copy(self) -> "DeviceTensor":
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.nsmi_tensor_copy(self.ctx.h, self.h, C.byref(h)))
+        return DeviceTensor(self.ctx, h)
+
+    def reshape_(self, shape):
+        dims = (C.c_int64 * max(len(shape), 1))(*shape)
+        self.ctx.check(self.ctx.lib.nsmi_tensor_reshape(self.ctx.h, self.h, len(shape), dims))
+        self._refresh()
+        return self
+
+    def scale_(self, a):
+        a = complex(a)
+        self.ctx.check(self.ctx.lib.nsmi_tensor_scale(self.ctx.h, self.h, a.real, a.imag))
+        return self
+
+    def axpy_(self, a, x: "DeviceTensor"):
+        a = complex(a)
+        self.ctx.check(self.ctx.lib.nsmi_tensor_axpy(self.ctx.h, a.real, a.imag, x.h, self.h))
+        return self
+
+    def norm(self) -> float:
+        v = C.c_double()
+        self.ctx.check(self.ctx.lib.nsmi_tensor_norm(self.ctx.h, self.h, C.byref(v)))
+        return v.value
+
+    def inner(self, other: "DeviceTensor"):
+        re, im = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx.lib.nsmi_tensor_inner(self.ctx.h, self.h, other.h, C.byref(re), C.byref(im)))
+        return re.value if self.dtype == np.float64 else complex(re.value, im.value)
+
+    def __repr__(self):
+        return f"DeviceTensor(shape={self.shape}, dtype={self.dtype})"

@@ -1,0 +1,127 @@
+"""Python wrappers of the region-step entry points of libnsmi.so."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+from . import _lib
+from .device import Context, DeviceTensor
+
+
+def _prec(p) -> int:
+    if isinstance(p, int):
+        return p
+    try:
+        return _lib.PREC[p]
+    except KeyError:
+        raise ValueError(f"unknown precision {p!r}; expected one of {sorted(_lib.PREC)}") from None
+
+
+def theta(A: DeviceTensor, B: DeviceTensor) -> DeviceTensor:
+    """theta[l,s1,s2,r] = sum_m A[l,s1,m] B[m,s2,r] (extracter.jl:7)."""
+    ctx = A.ctx
+    h = C.c_void_p()
+    ctx.check(ctx.lib.nsmi_theta(ctx.h, A.h, B.h, C.byref(h)))
+    return DeviceTensor(ctx, h)
+
+
+def env_update(E: DeviceTensor, A: DeviceTensor, W: DeviceTensor, side: int, precision="fp64") -> DeviceTensor:
+    """New environment across one site (side 0: left edge i->i+1, side 1: right edge i->i-1)."""
+    ctx = A.ctx
+    h = C.c_void_p()
+    ctx.check(ctx.lib.nsmi_env_update(ctx.h, E.h, A.h, W.h, int(side), _prec(precision), C.byref(h)))
+    return DeviceTensor(ctx, h)
+
+
+class HeffMap:
+    """Device effective Hamiltonian of a 1- or 2-site region (operator_map.jl:3-10).
+
+    Callable like the reference's ``psi -> optimal_map(P, psi)``: ``H(x)`` returns
+    a new device tensor.  Keeps its environment and MPO tensors alive."""
+
+    def __init__(self, L: DeviceTensor, W1: DeviceTensor, W2: DeviceTensor | None, R: DeviceTensor,
+                 precision="fp64"):
+        self.ctx = L.ctx
+        self._keep = (L, W1, W2, R)
+        self.precision = precision
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.nsmi_heff_create(self.ctx.h, L.h, W1.h, W2.h if W2 is not None else None,
+                                                     R.h, _prec(precision), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h is not None and not self.ctx.closed:
+            self.ctx.lib.nsmi_heff_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, x: DeviceTensor, y: DeviceTensor) -> DeviceTensor:
+        self.ctx.check(self.ctx.lib.nsmi_heff_apply(self.ctx.h, self.h, x.h, y.h))
+        return y
+
+    def __call__(self, x: DeviceTensor) -> DeviceTensor:
+        y = self.ctx.empty(x.shape, x.dtype)
+        return self.apply(x, y)
+
+    def rayleigh(self, x: DeviceTensor) -> float:
+        e = C.c_double()
+        self.ctx.check(self.ctx.lib.nsmi_heff_rayleigh(self.ctx.h, self.h, x.h, C.byref(e)))
+        return e.value
+
+
+@dataclass
+class LanczosResult:
+    eigval: float
+    numops: int
+    krylov_dim: int
+    normres: float
+
+
+def lanczos_eig(H: HeffMap, x: DeviceTensor, *, krylovdim=3, maxiter=1, tol=1e-14,
+                eta=1 / math.sqrt(2)) -> LanczosResult:
+    """Lowest Ritz pair of H; x is overwritten with the Ritz vector."""
+    opts = _lib.LanczosOpts(int(krylovdim), int(maxiter), float(tol), float(eta))
+    info = _lib.SolverInfo()
+    e = C.c_double()
+    H.ctx.check(H.ctx.lib.nsmi_lanczos_eig(H.ctx.h, H.h, x.h, C.byref(opts), C.byref(e), C.byref(info)))
+    return LanczosResult(e.value, info.numops, info.krylov_dim, info.normres)
+
+
+@dataclass
+class TruncInfo:
+    kept: int
+    full: int
+    truncerr: float
+    total: float
+    route: str
+
+
+def factorize(theta: DeviceTensor, n_row_dims: int, rows_trailing: bool, *, cutoff=0.0, maxdim=None,
+              mindim=1, which_decomp=None):
+    """Truncated split (ITensors factorize, inserter.jl:23); returns (U, C, TruncInfo)."""
+    ctx = theta.ctx
+    route = _lib.ROUTE["auto" if which_decomp is None else which_decomp]
+    t = _lib.Trunc(float(cutoff), int(maxdim) if maxdim is not None else 0, int(mindim), route)
+    U, Cc = C.c_void_p(), C.c_void_p()
+    info = _lib.TruncInfo()
+    ctx.check(ctx.lib.nsmi_factorize(ctx.h, theta.h, int(n_row_dims), int(bool(rows_trailing)), C.byref(t),
+                                     C.byref(U), C.byref(Cc), C.byref(info)))
+    return (DeviceTensor(ctx, U), DeviceTensor(ctx, Cc),
+            TruncInfo(info.kept, info.full, info.truncerr, info.total, _lib.ROUTE_NAME.get(info.route, "?")))
+
+
+def qr(A: DeviceTensor, n_row_dims: int, rows_trailing: bool):
+    """Thin QR with non-negative diag(R); returns (Q, R) in the factorize conventions."""
+    ctx = A.ctx
+    Q, R = C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.nsmi_qr(ctx.h, A.h, int(n_row_dims), int(bool(rows_trailing)), C.byref(Q), C.byref(R)))
+    return DeviceTensor(ctx, Q), DeviceTensor(ctx, R)
+
+
+__all__ = ["Context", "DeviceTensor", "theta", "env_update", "HeffMap", "lanczos_eig", "factorize", "qr"]

@@ -1,0 +1,85 @@
+"""CPU-side checks of the boundary: the C-ABI library loads, exports every symbol
+include/nsmi.h declares, and the Python signatures cover them (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nsmi.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char\*)\s+(nsmi_[a-z0-9_]+)\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from networksolvers_amd import _lib
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+    assert lib.nsmi_version() >= 10000
+
+
+def test_struct_layouts_match_header():
+    from networksolvers_amd import _lib
+    assert ctypes.sizeof(_lib.LanczosOpts) == 24
+    assert ctypes.sizeof(_lib.Trunc) == 32
+    assert ctypes.sizeof(_lib.TruncInfo) == 40
+    assert ctypes.sizeof(_lib.ProfileStats) == 32
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    from networksolvers_amd import _lib
+    lib = _lib.load()
+    assert lib.nsmi_ctx_create(0, 0, None) == -1
+    assert lib.nsmi_sync(None) == -1
+    assert lib.nsmi_tensor_free(None, None) == -1
+    assert lib.nsmi_last_error(None) == b"null context"
+
+
+def test_product_plans_match_oracle_plans():
+    from networksolvers_amd import region_plans as rp
+    from oracle import plans
+    verts = [(0, 0)] + [(b, s) for b in range(1, 4) for s in range(1, 4)]
+    edges = []
+    for b in range(1, 4):
+        edges.append(((0, 0), (b, 1)))
+        for s in range(2, 4):
+            edges.append(((b, s - 1), (b, s)))
+    G = rp.Graph(verts, edges)
+    for root in ((0, 0), (3, 3), (1, 2)):
+        assert rp.euler_tour_edges(G, root) == plans.euler_tour_edges(verts, edges, root)
+        assert rp.post_order_dfs_edges(G, root) == plans.post_order_dfs_edges(verts, edges, root)
+    for N in (2, 5, 9):
+        G = rp.path_graph(N)
+        assert [tuple(r) for r, _ in rp.euler_sweep(G, nsites=2)] == plans.euler_sweep_chain(N, N - 1)
+        assert rp.default_root_vertex(G) == N - 1
+    G = rp.path_graph(5)
+    got = rp.tdvp_regions(G, 0.1, updater_kwargs={}, tdvp_order=4, nsites=2, root_vertex=4)
+    want = plans.tdvp_regions(list(range(5)), [(i, i + 1) for i in range(4)], 4, 0.1, 4, 2)
+    assert [r for r, _ in got] == [r for r, _ in want]
+    assert np.allclose([k["updater_kwargs"]["time_step"] for _, k in got], [t for _, t in want])
+
+
+def test_truncation_parameters_schedule():
+    from networksolvers_amd.region_plans import truncation_parameters
+    t = [truncation_parameters(s, cutoff=1e-10, maxdim=[10, 20, 40, 64, 64]) for s in range(1, 7)]
+    assert [x["maxdim"] for x in t] == [10, 20, 40, 64, 64, 64]
+    assert all(x["cutoff"] == 1e-10 and x["mindim"] == 1 for x in t)
+
+
+def test_product_inputs_match_oracle_inputs():
+    from networksolvers_amd import models
+    from oracle import chain, models as om
+    for a, b in zip(models.xxz_mpo(7, Jz=0.3, hz=0.2), om.xxz_mpo(7, Jz=0.3, hz=0.2)):
+        assert np.array_equal(a, b)
+    for a, b in zip(models.xxz_mpo(5, d=3), om.xxz_mpo(5, d=3)):
+        assert np.array_equal(a, b)
+    for a, b in zip(models.random_mps(9, 2, 6, seed=5), chain.random_mps(9, 2, 6, seed=5)):
+        np.testing.assert_allclose(a, b, rtol=1e-15, atol=0)

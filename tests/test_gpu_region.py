@@ -1,0 +1,222 @@
+"""GPU parity of the region-step kernels (through the C ABI) against the oracle.
+
+Tolerances: FP64 contractions agree with the oracle to 1e-12 relative (different
+summation order only); the FP32 form to 2e-5 relative (FP32 operands, FP32
+accumulation); factorize/QR agree after the shared gauge convention to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from oracle import chain, linalg, models
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import networksolvers_amd as ns
+    c = ns.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def rand(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape)
+
+
+@pytest.mark.parametrize("shape", [(1, 2, 3, 2, 1), (5, 2, 7, 2, 3), (37, 3, 64, 3, 29), (130, 2, 129, 2, 131)])
+def test_theta(ctx, shape):
+    import networksolvers_amd as ns
+    cl, d1, cm, d2, cr = shape
+    A, B = rand((cl, d1, cm), 1), rand((cm, d2, cr), 2)
+    t = ns.theta(ctx.upload(A), ctx.upload(B)).numpy()
+    assert t.shape == (cl, d1, d2, cr)
+    assert rel(t, chain.theta2(A, B)) < 1e-13
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("dims", [(1, 2, 1, 1, 5), (3, 2, 4, 5, 5), (33, 2, 17, 5, 5), (70, 3, 90, 5, 5),
+                                  (130, 2, 200, 5, 5), (64, 2, 64, 5, 1), (64, 2, 64, 1, 5)])
+def test_env_update(ctx, side, dims):
+    import networksolvers_amd as ns
+    ca, d, cb, Dl, Dr = dims
+    A = rand((ca, d, cb), 3)
+    W = rand((Dl, Dr, d, d), 4)
+    if side == 0:
+        E = rand((ca, Dl, ca), 5)
+        want = chain.env_left(E, A, W)
+    else:
+        E = rand((cb, Dr, cb), 5)
+        want = chain.env_right(E, A, W)
+    got = ns.env_update(ctx.upload(E), ctx.upload(A), ctx.upload(W), side).numpy()
+    assert got.shape == want.shape
+    assert rel(got, want) < 1e-12
+
+
+def _heff_case(cl, cr, d, D, seed):
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal((cl, D, cl))
+    R = rng.standard_normal((cr, D, cr))
+    W1 = rng.standard_normal((D, D, d, d))
+    W2 = rng.standard_normal((D, D, d, d))
+    th = rng.standard_normal((cl, d, d, cr))
+    return L, W1, W2, R, th
+
+
+@pytest.mark.parametrize("cl,cr,d,D", [(1, 1, 2, 1), (4, 3, 2, 5), (20, 33, 2, 5), (64, 64, 2, 5),
+                                       (100, 70, 3, 5), (129, 257, 2, 3)])
+def test_heff2_fp64(ctx, cl, cr, d, D):
+    import networksolvers_amd as ns
+    L, W1, W2, R, th = _heff_case(cl, cr, d, D, 7)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W1), ctx.upload(W2), ctx.upload(R), "fp64")
+    got = H(ctx.upload(th)).numpy()
+    assert rel(got, chain.heff2_apply(L, W1, W2, R, th)) < 1e-12
+
+
+def test_heff2_matches_dense_matrix(ctx):
+    import networksolvers_amd as ns
+    W = models.xxz_mpo(6, Jz=0.7, hz=0.3)
+    mps = chain.random_mps(6, 2, 3, seed=3)
+    L = np.ones((1, 1, 1))
+    for i in range(2):
+        L = chain.env_left(L, mps[i], W[i])
+    R = np.ones((1, 1, 1))
+    for i in range(5, 3, -1):
+        R = chain.env_right(R, mps[i], W[i])
+    th = chain.theta2(mps[2], mps[3])
+    Hd = chain.heff2_dense_matrix(L, W[2], W[3], R)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[2]), ctx.upload(W[3]), ctx.upload(R))
+    got = H(ctx.upload(th)).numpy().reshape(-1)
+    np.testing.assert_allclose(got, Hd @ th.reshape(-1), atol=1e-13)
+
+
+@pytest.mark.parametrize("cl,cr,d,D", [(3, 5, 2, 5), (50, 40, 2, 5), (128, 128, 3, 5)])
+def test_heff1_fp64(ctx, cl, cr, d, D):
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(11)
+    L, R = rng.standard_normal((cl, D, cl)), rng.standard_normal((cr, D, cr))
+    W = rng.standard_normal((D, D, d, d))
+    x = rng.standard_normal((cl, d, cr))
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W), None, ctx.upload(R))
+    assert rel(H(ctx.upload(x)).numpy(), chain.heff1_apply(L, W, R, x)) < 1e-12
+
+
+@pytest.mark.parametrize("cl,cr", [(16, 16), (96, 80)])
+def test_heff2_fp32(ctx, cl, cr):
+    import networksolvers_amd as ns
+    L, W1, W2, R, th = _heff_case(cl, cr, 2, 5, 9)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W1), ctx.upload(W2), ctx.upload(R), "fp32")
+    got = H(ctx.upload(th)).numpy()
+    assert rel(got, chain.heff2_apply(L, W1, W2, R, th)) < 2e-5
+
+
+def _physical_heff(N, chi, seed, lo):
+    W = models.xxz_mpo(N, Jz=1.0)
+    mps = chain.random_mps(N, 2, chi, seed=seed)
+    mps = chain.orthogonalize_chain(mps, lo, lo + 1)
+    L = np.ones((1, 1, 1))
+    for i in range(lo):
+        L = chain.env_left(L, mps[i], W[i])
+    R = np.ones((1, 1, 1))
+    for i in range(N - 1, lo + 1, -1):
+        R = chain.env_right(R, mps[i], W[i])
+    return W, mps, L, R
+
+
+@pytest.mark.parametrize("krylovdim", [1, 2, 3, 5])
+def test_lanczos_matches_oracle(ctx, krylovdim):
+    import networksolvers_amd as ns
+    N, lo = 10, 4
+    W, mps, L, R = _physical_heff(N, 12, 21, lo)
+    th = chain.theta2(mps[lo], mps[lo + 1])
+    shape = th.shape
+    e0, x0, info0 = linalg.lanczos_eigsolve(
+        lambda t: chain.heff2_apply(L, W[lo], W[lo + 1], R, t.reshape(shape)).reshape(-1),
+        th.reshape(-1), krylovdim=krylovdim)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[lo]), ctx.upload(W[lo + 1]), ctx.upload(R))
+    x = ctx.upload(th)
+    res = ns.lanczos_eig(H, x, krylovdim=krylovdim)
+    assert res.numops == info0.numops and res.krylov_dim == info0.krylov_dim
+    assert abs(res.eigval - e0) < 1e-12 * abs(e0)
+    assert rel(x.numpy().reshape(-1), x0) < 1e-10
+
+
+def test_lanczos_invariant_subspace_stop(ctx):
+    """Start vector = exact eigenvector of a tiny H_eff: beta <= tol after one vector."""
+    import networksolvers_amd as ns
+    W = models.xxz_mpo(2)
+    L = np.ones((1, 1, 1))
+    R = np.ones((1, 1, 1))
+    Hd = chain.heff2_dense_matrix(L, W[0], W[1], R)
+    w, v = np.linalg.eigh(Hd)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[0]), ctx.upload(W[1]), ctx.upload(R))
+    x = ctx.upload(v[:, 0].reshape(1, 2, 2, 1))
+    res = ns.lanczos_eig(H, x, krylovdim=3, tol=1e-12)
+    assert res.krylov_dim == 1 and res.numops == 1
+    assert abs(res.eigval - w[0]) < 1e-13
+
+
+@pytest.mark.parametrize("route", ["svd", "eigen"])
+@pytest.mark.parametrize("trailing", [False, True])
+@pytest.mark.parametrize("shape,cutoff,maxdim", [((4, 2, 2, 6), 1e-10, None), ((20, 2, 2, 24), 1e-6, 30),
+                                                  ((64, 2, 2, 64), 1e-9, 64), ((1, 2, 2, 1), 0.0, None),
+                                                  ((50, 3, 3, 10), 1e-12, 20)])
+def test_factorize(ctx, route, trailing, shape, cutoff, maxdim):
+    import networksolvers_amd as ns
+    cl, d1, d2, cr = shape
+    rng = np.random.default_rng(13)
+    # graded spectrum: product of random isometries and decaying values
+    m, n = cl * d1, d2 * cr
+    r = min(m, n)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = np.exp(-np.arange(r) / 6.0)
+    th = ((U0 * s) @ V0.T).reshape(shape)
+    M = th.reshape(m, n)
+    U, Cm, info = ns.factorize(ctx.upload(th), 2, trailing, cutoff=cutoff, maxdim=maxdim, which_decomp=route)
+    if trailing:
+        Uo, Co, io = linalg.factorize(M.T, cutoff=cutoff, maxdim=maxdim, which_decomp=route)
+        Ug, Cg = U.numpy().reshape(-1, n).T, Cm.numpy().reshape(m, -1).T
+    else:
+        Uo, Co, io = linalg.factorize(M, cutoff=cutoff, maxdim=maxdim, which_decomp=route)
+        Ug, Cg = U.numpy().reshape(m, -1), Cm.numpy().reshape(-1, n)
+    assert info.kept == io.kept and info.route == io.route
+    assert abs(info.truncerr - io.truncerr) < 1e-12 + 1e-8 * io.truncerr
+    # the eigen route resolves eigenvectors of small eigenvalues only to eps/gap
+    tol = 1e-10 if route == "svd" else 1e-8
+    np.testing.assert_allclose(Ug, Uo, atol=tol)
+    np.testing.assert_allclose(Cg, Co, atol=tol)
+    np.testing.assert_allclose(Ug @ Cg, Uo @ Co, atol=1e-12)
+
+
+@pytest.mark.parametrize("trailing", [False, True])
+@pytest.mark.parametrize("shape", [(3, 2, 5), (8, 2, 3), (16, 2, 16), (1, 2, 4), (40, 3, 100)])
+def test_qr(ctx, trailing, shape):
+    import networksolvers_amd as ns
+    A = rand(shape, 17)
+    Q, R = ns.qr(ctx.upload(A), 2 if not trailing else 1, trailing)
+    if not trailing:
+        M = A.reshape(shape[0] * shape[1], shape[2])
+        Qo, Ro = linalg.qr_pos(M)
+        np.testing.assert_allclose(Q.numpy().reshape(Qo.shape), Qo, atol=1e-12)
+        np.testing.assert_allclose(R.numpy().reshape(Ro.shape), Ro, atol=1e-12)
+    else:
+        M = A.reshape(shape[0], shape[1] * shape[2])
+        Qo, Ro = linalg.qr_pos(M.T)
+        np.testing.assert_allclose(Q.numpy().reshape(-1, M.shape[1]), Qo.T, atol=1e-12)
+        np.testing.assert_allclose(R.numpy().reshape(M.shape[0], -1), Ro.T, atol=1e-12)
+
+
+def test_errors_are_reported_not_crashed(ctx):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    A = ctx.upload(rand((3, 2, 4), 1))
+    B = ctx.upload(rand((5, 2, 4), 2))
+    with pytest.raises(NsmiError, match="theta"):
+        ns.theta(A, B)
+    with pytest.raises(NsmiError):
+        ns.factorize(ctx.upload(rand((2, 2, 2, 2), 3)), 0, False)
