@@ -6,7 +6,8 @@
 // LDS slabs of 16 with register prefetch of the next slab (double-buffered LDS,
 // one barrier per slab).  Lane layouts of the 16x16x4 instructions:
 //   A operand: lane l holds A[l%16][l/16]    B operand: lane l holds B[l/16][l%16]
-//   D:         lane l, item r holds D[4*(l/16)+r][l%16]
+//   D (f32):   lane l, item r holds D[4*(l/16)+r][l%16]
+//   D (f64):   lane l, item r holds D[l/16+4*r][l%16]
 // BF16: v_mfma_f32_16x16x32_bf16 -- A: lane l holds A[l%16][8*(l/16)+j], j<8;
 // B: B[8*(l/16)+j][l%16]; D as above.  LDS holds K-contiguous rows so every
 // fragment is one 16-byte LDS read.
@@ -31,6 +32,13 @@ __device__ inline d4 mfma16(double a, double b, d4 c) {
 __device__ inline f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+// Row of item r of lane group kl = lane/16 in a 16x16 MFMA result block
+// (measured on gfx950): f64 16x16x4 interleaves rows kl + 4r; f32 16x16x4
+// and bf16 16x16x32 hold four consecutive rows 4*kl + r.
+template <typename T> __device__ inline int out_row(int kl, int r);
+template <> __device__ inline int out_row<double>(int kl, int r) { return kl + 4 * r; }
+template <> __device__ inline int out_row<float>(int kl, int r) { return 4 * kl + r; }
 
 template <typename T>
 struct GemmArgs {
@@ -147,7 +155,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T> g) {
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm * WTM + i * 16 + kl * 4 + r;
+        const int64_t gm = m0 + wm * WTM + i * 16 + out_row<T>(kl, r);
         const int64_t gn = n0 + wn * WTN + j * 16 + ml;
         if (gm < g.M && gn < g.N) {
           T v = g.alpha * acc[i][j][r];
