@@ -55,8 +55,18 @@ def run_sweeps(prob, nsweeps, maxdim, cutoff, precision):
     """Continues the same problem (state and cached environments) for nsweeps."""
     from networksolvers_amd.sweep import eigsolve_problem
     nlog = len(prob.region_log)
+    t_last = [time.perf_counter()]
+
+    def progress(region_iter, *, sweep, nsweeps, **kw):
+        now = time.perf_counter()
+        p = region_iter.problem
+        print(f"bench:   sweep {sweep}/{nsweeps} ({precision}) {now - t_last[0]:.2f} s  E={p.eigenvalue:.12f}  "
+              f"maxlinkdim={p.state.maxlinkdim()}", file=sys.stderr, flush=True)
+        t_last[0] = now
+
     E, psi = eigsolve_problem(prob, nsweeps=nsweeps, nsites=2, updater_kwargs=dict(precision=precision),
-                              inserter_kwargs=dict(trunc=dict(cutoff=cutoff, maxdim=maxdim)))
+                              inserter_kwargs=dict(trunc=dict(cutoff=cutoff, maxdim=maxdim)),
+                              sweep_printer=progress)
     return E, psi, prob.region_log[nlog:]
 
 

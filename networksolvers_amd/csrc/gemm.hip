@@ -224,3 +224,145 @@ void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, 
 }
 
 }  // namespace nsmi
+
+// ---------------------------------------------------------------------------
+// BF16 MFMA GEMM, "NT" form: C[M x N] = sum_t A_t[M x Kp] . B_t[N x Kp]^T with
+// both operands K-contiguous bf16 planes (row pitch Kp, Kp % 32 == 0, padding
+// zero).  TERMS == 3 computes Ahi.Bhi + Ahi.Blo + Alo.Bhi (bf16x3 split
+// products, FP32 accumulation); TERMS == 1 computes Ahi.Bhi.
+// 256 threads = 4 waves (2 x 2), 128 x 128 tile, K slab 32 (one MFMA k-step),
+// LDS rows padded to 80 bytes so the 16 lanes of each quarter-wave hit 16
+// distinct 4-bank groups on their 16-byte fragment reads.
+// ---------------------------------------------------------------------------
+namespace nsmi {
+namespace {
+
+struct BfArgs {
+  const uint16_t *Ahi, *Alo, *Bhi, *Blo;
+  void* C;
+  int64_t M, N, Kp, lda, ldb, ldc;
+  int tiles_m, tiles_n;
+};
+
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+template <int TERMS, typename TC, int ROLE>
+__global__ __launch_bounds__(256) void k_gemm_bf16_nt(BfArgs g) {
+  constexpr int BM = 128, BN = 128, BK = 32, PITCH = BK + 8;  // in bf16 elements
+  constexpr int NP = (TERMS == 3) ? 2 : 1;                     // planes per operand
+  __shared__ __attribute__((aligned(16))) uint16_t sA[2][NP][BM][PITCH];
+  __shared__ __attribute__((aligned(16))) uint16_t sB[2][NP][BN][PITCH];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int nwg = g.tiles_m * g.tiles_n;
+  const int tile = xcd_remap(blockIdx.x, nwg);
+  const int tm = tile / g.tiles_n, tn = tile % g.tiles_n;
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+
+  // each thread stages one row-half (16 bf16 = 2 x 16 B) of every tile plane
+  const int lr = tid >> 1, lh = (tid & 1) * 16;
+  const bool a_ok = (m0 + lr) < g.M, b_ok = (n0 + lr) < g.N;
+  const uint16_t* pa[2] = {g.Ahi + (m0 + lr) * g.lda + lh, TERMS == 3 ? g.Alo + (m0 + lr) * g.lda + lh : nullptr};
+  const uint16_t* pb[2] = {g.Bhi + (n0 + lr) * g.ldb + lh, TERMS == 3 ? g.Blo + (n0 + lr) * g.ldb + lh : nullptr};
+  u4 ra[NP][2], rb[NP][2];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        ra[p][h] = a_ok ? *reinterpret_cast<const u4*>(pa[p] + k0 + 8 * h) : u4{0, 0, 0, 0};
+        rb[p][h] = b_ok ? *reinterpret_cast<const u4*>(pb[p] + k0 + 8 * h) : u4{0, 0, 0, 0};
+      }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        *reinterpret_cast<u4*>(&sA[buf][p][lr][lh + 8 * h]) = ra[p][h];
+        *reinterpret_cast<u4*>(&sB[buf][p][lr][lh + 8 * h]) = rb[p][h];
+      }
+  };
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+  const int64_t nk = g.Kp / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int buf = int(kt & 1);
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    bf8 ah[4], bh[4], al[4], bl[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ah[i] = *reinterpret_cast<const bf8*>(&sA[buf][0][wm * 64 + i * 16 + fr][fk]);
+      bh[i] = *reinterpret_cast<const bf8*>(&sB[buf][0][wn * 64 + i * 16 + fr][fk]);
+      if constexpr (TERMS == 3) {
+        al[i] = *reinterpret_cast<const bf8*>(&sA[buf][1][wm * 64 + i * 16 + fr][fk]);
+        bl[i] = *reinterpret_cast<const bf8*>(&sB[buf][1][wn * 64 + i * 16 + fr][fk]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (TERMS == 3) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        }
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+  TC* C = static_cast<TC*>(g.C);
+  const int kl = lane >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * 64 + i * 16 + 4 * kl + r;  // bf16 16x16x32: rows 4*(l/16)+r
+        const int64_t gn = n0 + wn * 64 + j * 16 + fr;
+        if (gm < g.M && gn < g.N) C[gm * g.ldc + gn] = TC(acc[i][j][r]);
+      }
+}
+
+template <int TERMS, typename TC>
+void bf16_role(nsmi_ctx* ctx, int role, dim3 grid, const BfArgs& a) {
+  switch (role) {
+    case kRoleHeff1: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleHeff1>), grid, dim3(256), 0, ctx->stream, a); break;
+    case kRoleHeff2: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleHeff2>), grid, dim3(256), 0, ctx->stream, a); break;
+    default: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleEnv>), grid, dim3(256), 0, ctx->stream, a); break;
+  }
+}
+
+}  // namespace
+
+void gemm_bf16_nt(nsmi_ctx* ctx, int64_t M, int64_t N, int64_t Kp, const uint16_t* Ahi, const uint16_t* Alo,
+                  int64_t lda, const uint16_t* Bhi, const uint16_t* Blo, int64_t ldb, void* C, int64_t ldc,
+                  bool c_double, int terms, int role) {
+  require(Kp % 32 == 0 && lda % 8 == 0 && ldb % 8 == 0, "gemm_bf16: K must be padded to 32, rows 16-byte aligned");
+  require(terms == 1 || terms == 3, "gemm_bf16: terms must be 1 or 3");
+  if (M == 0 || N == 0) return;
+  BfArgs a{Ahi, Alo, Bhi, Blo, C, M, N, Kp, lda, ldb, ldc, int((M + 127) / 128), int((N + 127) / 128)};
+  dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n));
+  if (terms == 3) {
+    if (c_double) bf16_role<3, double>(ctx, role, grid, a);
+    else bf16_role<3, float>(ctx, role, grid, a);
+  } else {
+    if (c_double) bf16_role<1, double>(ctx, role, grid, a);
+    else bf16_role<1, float>(ctx, role, grid, a);
+  }
+  NSMI_LAUNCH_CHECK();
+}
+
+}  // namespace nsmi

@@ -145,9 +145,20 @@ void gemm_f64(nsmi_ctx* ctx, const GemmDesc& g, double alpha, const double* A, c
               double beta, double* C, int role);
 // FP32 MFMA on float operands, FP32 result
 void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, float* C, int role);
-// BF16 MFMA; operands pre-split into hi (and lo) planes; terms = 1 or 3; FP32 result
-void gemm_bf16(nsmi_ctx* ctx, const GemmDesc& g, const uint16_t* Ahi, const uint16_t* Alo,
-               const uint16_t* Bhi, const uint16_t* Blo, float* C, int terms, int role);
+// BF16 MFMA, C[M x N] = A[M x Kp] . B[N x Kp]^T on pre-split bf16 hi (and lo)
+// planes (terms = 1: hi.hi; 3: hi.hi + hi.lo + lo.hi); Kp % 32 == 0 with zero
+// padding; C is FP32 or FP64 (c_double)
+void gemm_bf16_nt(nsmi_ctx* ctx, int64_t M, int64_t N, int64_t Kp, const uint16_t* Ahi, const uint16_t* Alo,
+                  int64_t lda, const uint16_t* Bhi, const uint16_t* Blo, int64_t ldb, void* C, int64_t ldc,
+                  bool c_double, int terms, int role);
+inline int64_t pad32(int64_t k) { return (k + 31) / 32 * 32; }
+// split an FP64 matrix (rows x K, row stride K) into bf16 hi/lo planes of pitch Kp (zero padded);
+// transpose != 0 reads the source as K x rows (row stride rows) instead
+void split_bf16_pad(nsmi_ctx* ctx, const double* X, int64_t rows, int64_t K, int64_t Kp, uint16_t* hi,
+                    uint16_t* lo, int transpose);
+// H_eff mixing into GEMM2's bf16 operand: T3[(o*S + srow), v*I + i] for p = srow*D2 + v
+void mix_f32_to_bf16_rows(nsmi_ctx* ctx, const float* X, uint16_t* Yhi, uint16_t* Ylo, const double* Mx,
+                          int64_t O, int P, int Q, int64_t I, int D2, int64_t Kp, int cls);
 
 // roles select distinct kernel instantiations so each call site is a separate
 // kernel name in profiles
@@ -180,9 +191,6 @@ void mix_f64(nsmi_ctx* ctx, const double* X, double* Y, const double* Mx, int64_
              int64_t I, int cls);
 void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t O, int P, int Q,
              int64_t I, int cls);
-// mixing whose output is written as bf16 hi/lo planes (feeds a BF16 GEMM)
-void mix_f32_to_bf16(nsmi_ctx* ctx, const float* X, uint16_t* Yhi, uint16_t* Ylo, const double* Mx,
-                     int64_t O, int P, int Q, int64_t I, int cls);
 
 void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jacobi, ascending
 

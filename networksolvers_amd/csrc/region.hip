@@ -27,7 +27,11 @@ struct nsmi_heff {
   void* T3 = nullptr;    // mixing output
   float* L32 = nullptr;
   float* R32 = nullptr;
+  // BF16 forms: operand planes (K padded to 32), per-apply workspace
   uint16_t *Lhi = nullptr, *Llo = nullptr, *Rhi = nullptr, *Rlo = nullptr;
+  uint16_t *xThi = nullptr, *xTlo = nullptr, *T3hi = nullptr, *T3lo = nullptr;
+  float* T1f = nullptr;
+  int64_t Kp1 = 0, Kp2 = 0;
   double flops_gemm1 = 0, flops_gemm2 = 0, flops_mix = 0;
   int64_t n = 0;  // elements of theta
 };
@@ -139,6 +143,78 @@ __global__ void k_build_w1(const double* __restrict__ W, double* __restrict__ Mx
   }
 }
 
+// round-to-nearest-even FP32 -> BF16 bits (finite inputs)
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__device__ inline float bf16_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// hi = bf16(v), lo = bf16(v - hi) for every (r, k < Kp); zero beyond K.
+__global__ void k_split_rows(const double* __restrict__ X, int64_t rows, int64_t K, int64_t Kp,
+                             uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * Kp) return;
+  const int64_t r = t / Kp, k = t % Kp;
+  const double v = k < K ? X[r * K + k] : 0.0;
+  const uint16_t h = bf16_rne(float(v));
+  hi[t] = h;
+  if (lo) lo[t] = bf16_rne(float(v - double(bf16_f32(h))));
+}
+
+// transposing split: source X is K x rows (row stride rows); out[r][k] = X[k][r]
+__global__ void k_split_transpose(const double* __restrict__ X, int64_t rows, int64_t K, int64_t Kp,
+                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+  __shared__ double tile[32][33];
+  const int64_t r0 = int64_t(blockIdx.x) * 32, k0 = int64_t(blockIdx.y) * 32;
+  for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+    const int64_t k = k0 + j, r = r0 + threadIdx.x;
+    tile[j][threadIdx.x] = (k < K && r < rows) ? X[k * rows + r] : 0.0;
+  }
+  __syncthreads();
+  for (int j = threadIdx.y; j < 32; j += blockDim.y) {
+    const int64_t r = r0 + j, k = k0 + threadIdx.x;
+    if (r < rows && k < Kp) {
+      const double v = tile[threadIdx.x][j];
+      const uint16_t h = bf16_rne(float(v));
+      hi[r * Kp + k] = h;
+      if (lo) lo[r * Kp + k] = bf16_rne(float(v - double(bf16_f32(h))));
+    }
+  }
+}
+
+// H_eff mixing written as GEMM2's bf16 operand:
+// Y[(o*S + p/D2), (p%D2)*I + i] = sum_q Mx[p][q] X[o][q][i],  S = P / D2
+template <int QMAX>
+__global__ __launch_bounds__(256) void k_mix_bf16_rows(const float* __restrict__ X, uint16_t* __restrict__ Yhi,
+                                                       uint16_t* __restrict__ Ylo, const double* __restrict__ Mx,
+                                                       int64_t O, int P, int Q, int64_t I, int D2, int64_t Kp) {
+  __shared__ float sM[64 * 64];
+  for (int t = threadIdx.x; t < P * Q; t += blockDim.x) sM[t] = float(Mx[t]);
+  __syncthreads();
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= I) return;
+  const int S = P / D2;
+  for (int64_t o = blockIdx.y; o < O; o += gridDim.y) {
+    const float* xo = X + o * Q * I + i;
+    float x[QMAX];
+#pragma unroll
+    for (int q = 0; q < QMAX; ++q) x[q] = (q < Q) ? xo[int64_t(q) * I] : 0.f;
+    for (int p = 0; p < P; ++p) {
+      const float* m = sM + p * Q;
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < QMAX; ++q)
+        if (q < Q) acc += m[q] * x[q];
+      const int64_t idx = (o * S + p / D2) * Kp + int64_t(p % D2) * I + i;
+      const uint16_t h = bf16_rne(acc);
+      Yhi[idx] = h;
+      if (Ylo) Ylo[idx] = bf16_rne(acc - bf16_f32(h));
+    }
+  }
+}
+
 void build_w1(nsmi_ctx* ctx, const double* W, double* Mx, int Dl, int Dr, int d, int mode) {
   const int n = Dl * Dr * d * d;
   hipLaunchKernelGGL(k_build_w1, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, W, Mx, Dl, Dr, d, mode);
@@ -154,6 +230,34 @@ void mix_f64(nsmi_ctx* ctx, const double* X, double* Y, const double* Mx, int64_
 void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t O, int P, int Q, int64_t I,
              int cls) {
   mix_launch<float, float>(ctx, X, Y, Mx, O, P, Q, I, cls);
+}
+
+void split_bf16_pad(nsmi_ctx* ctx, const double* X, int64_t rows, int64_t K, int64_t Kp, uint16_t* hi, uint16_t* lo,
+                    int transpose) {
+  if (rows == 0 || Kp == 0) return;
+  if (!transpose) {
+    const int64_t n = rows * Kp;
+    hipLaunchKernelGGL(k_split_rows, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream, X, rows, K, Kp, hi, lo);
+  } else {
+    dim3 grid(unsigned((rows + 31) / 32), unsigned((Kp + 31) / 32));
+    hipLaunchKernelGGL(k_split_transpose, grid, dim3(32, 8), 0, ctx->stream, X, rows, K, Kp, hi, lo);
+  }
+  NSMI_LAUNCH_CHECK();
+}
+
+void mix_f32_to_bf16_rows(nsmi_ctx* ctx, const float* X, uint16_t* Yhi, uint16_t* Ylo, const double* Mx, int64_t O,
+                          int P, int Q, int64_t I, int D2, int64_t Kp, int cls) {
+  require(P >= 1 && Q >= 1 && P <= 64 && Q <= 64 && D2 >= 1 && P % D2 == 0, "mix_bf16: bad P, Q, D2");
+  if (O == 0 || I == 0) return;
+  ProfScope ps(ctx, cls, 2.0 * double(O) * double(I) * P * Q, double(O) * double(I) * (P * 4.0 + Q * 4.0));
+  dim3 grid(unsigned((I + 255) / 256), unsigned(std::min<int64_t>(O, 65535)));
+  if (Q <= 16)
+    hipLaunchKernelGGL((k_mix_bf16_rows<16>), grid, dim3(256), 0, ctx->stream, X, Yhi, Ylo, Mx, O, P, Q, I, D2, Kp);
+  else if (Q <= 32)
+    hipLaunchKernelGGL((k_mix_bf16_rows<32>), grid, dim3(256), 0, ctx->stream, X, Yhi, Ylo, Mx, O, P, Q, I, D2, Kp);
+  else
+    hipLaunchKernelGGL((k_mix_bf16_rows<64>), grid, dim3(256), 0, ctx->stream, X, Yhi, Ylo, Mx, O, P, Q, I, D2, Kp);
+  NSMI_LAUNCH_CHECK();
 }
 
 // Cyclic Jacobi eigensolver for small dense symmetric matrices (host), used for
@@ -248,7 +352,32 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     vec_f32_to_f64(ctx, y, ys.as<float>(), op->n);
     return;
   }
+  if (prec == NSMI_PREC_BF16X3 || prec == NSMI_PREC_BF16) {
+    // theta^T split into bf16 planes: GEMM1 is L . theta as an NT product
+    const int terms = prec == NSMI_PREC_BF16X3 ? 3 : 1;
+    split_bf16_pad(ctx, x, N1, K1, op->Kp1, op->xThi, op->xTlo, 1);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 2.0 * terms * (M1 + N1) * op->Kp1 + 4.0 * M1 * N1);
+      gemm_bf16_nt(ctx, M1, N1, op->Kp1, op->Lhi, op->Llo, op->Kp1, op->xThi, op->xTlo, op->Kp1, op->T1f, N1,
+                   false, terms, kRoleHeff1);
+    }
+    mix_f32_to_bf16_rows(ctx, op->T1f, op->T3hi, op->T3lo, op->Mx, op->cl, op->P, op->Q, op->cr, int(op->D2),
+                         op->Kp2, NSMI_K_HEFF_MIX);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 2.0 * terms * (M2 + N2) * op->Kp2 + 8.0 * M2 * N2);
+      gemm_bf16_nt(ctx, M2, N2, op->Kp2, op->T3hi, op->T3lo, op->Kp2, op->Rhi, op->Rlo, op->Kp2, y, N2, true, terms,
+                   kRoleHeff2);
+    }
+    return;
+  }
   fail(NSMI_ERR_UNSUPPORTED, "H_eff precision not served by this build");
+}
+
+static void heff_release(nsmi_ctx* ctx, nsmi_heff* op) {
+  void* bufs[] = {op->Mx, op->T1, op->T3, op->L32, op->R32, op->Lhi, op->Llo, op->Rhi, op->Rlo,
+                  op->xThi, op->xTlo, op->T3hi, op->T3lo, op->T1f};
+  for (void* b : bufs) scratch_release(ctx, b);
+  delete op;
 }
 
 extern "C" {
@@ -279,8 +408,6 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32 || precision == NSMI_PREC_BF16X3 ||
               precision == NSMI_PREC_BF16,
           "heff: unknown precision");
-  if (precision == NSMI_PREC_BF16X3 || precision == NSMI_PREC_BF16)
-    fail(NSMI_ERR_UNSUPPORTED, "heff: BF16 precisions not served by this build");
   op = new nsmi_heff();
   op->L = L;
   op->R = R;
@@ -319,9 +446,35 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
     build_w1(ctx, dptr(W1), op->Mx, int(op->D0), int(op->D2), int(d1), 0);
   }
   op->n = op->cl * op->dphys * op->cr;
-  const size_t elt = prec_elt(precision);
-  op->T1 = scratch(ctx, elt * size_t(op->cl * op->D0 * op->dphys * op->cr));
-  op->T3 = scratch(ctx, elt * size_t(op->cl * op->D2 * op->dphys * op->cr));
+  const bool bf = precision == NSMI_PREC_BF16X3 || precision == NSMI_PREC_BF16;
+  const bool two = precision == NSMI_PREC_BF16X3;
+  if (!bf) {
+    const size_t elt = prec_elt(precision);
+    op->T1 = scratch(ctx, elt * size_t(op->cl * op->D0 * op->dphys * op->cr));
+    op->T3 = scratch(ctx, elt * size_t(op->cl * op->D2 * op->dphys * op->cr));
+  } else {
+    // planes: L as (cl*D0) x Kp1, theta^T as (dphys*cr) x Kp1, T3 as (cl*dphys) x Kp2, R as cr x Kp2
+    op->Kp1 = pad32(op->cl);
+    op->Kp2 = pad32(op->D2 * op->cr);
+    const size_t nL = size_t(op->cl * op->D0 * op->Kp1), nX = size_t(op->dphys * op->cr * op->Kp1);
+    const size_t nT3 = size_t(op->cl * op->dphys * op->Kp2), nR = size_t(op->cr * op->Kp2);
+    op->Lhi = static_cast<uint16_t*>(scratch(ctx, 2 * nL));
+    op->Rhi = static_cast<uint16_t*>(scratch(ctx, 2 * nR));
+    op->xThi = static_cast<uint16_t*>(scratch(ctx, 2 * nX));
+    op->T3hi = static_cast<uint16_t*>(scratch(ctx, 2 * nT3));
+    if (two) {
+      op->Llo = static_cast<uint16_t*>(scratch(ctx, 2 * nL));
+      op->Rlo = static_cast<uint16_t*>(scratch(ctx, 2 * nR));
+      op->xTlo = static_cast<uint16_t*>(scratch(ctx, 2 * nX));
+      op->T3lo = static_cast<uint16_t*>(scratch(ctx, 2 * nT3));
+    }
+    op->T1f = static_cast<float*>(scratch(ctx, 4 * size_t(op->cl * op->D0 * op->dphys * op->cr)));
+    split_bf16_pad(ctx, dptr(L), op->cl * op->D0, op->cl, op->Kp1, op->Lhi, op->Llo, 0);
+    split_bf16_pad(ctx, dptr(R), op->cr, op->D2 * op->cr, op->Kp2, op->Rhi, op->Rlo, 0);
+    // the K padding of T3 is never written by the mixing kernel: clear it once
+    NSMI_HIP(hipMemsetAsync(op->T3hi, 0, 2 * nT3, ctx->stream));
+    if (two) NSMI_HIP(hipMemsetAsync(op->T3lo, 0, 2 * nT3, ctx->stream));
+  }
   op->flops_gemm1 = 2.0 * double(op->cl * op->D0) * double(op->dphys * op->cr) * double(op->cl);
   op->flops_gemm2 = 2.0 * double(op->cl * op->dphys) * double(op->cr) * double(op->D2 * op->cr);
   op->flops_mix = 2.0 * double(op->cl) * double(op->cr) * op->P * op->Q;
@@ -333,23 +486,25 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   }
   *out = op;
   op = nullptr;
-  API_END(ctx)
+  }
+  catch (const nsmi::Error& e) {
+    if (op) heff_release(ctx, op);
+    ctx->err = e.what();
+    return e.code;
+  }
+  catch (const std::exception& e) {
+    if (op) heff_release(ctx, op);
+    ctx->err = std::string("internal: ") + e.what();
+    return NSMI_ERR_INVALID;
+  }
+  return NSMI_OK;
 }
 
 int nsmi_heff_destroy(nsmi_ctx* ctx, nsmi_heff* op) {
   if (!ctx) return NSMI_ERR_INVALID;
   if (!op) return NSMI_OK;
   API_BEGIN
-  scratch_release(ctx, op->Mx);
-  scratch_release(ctx, op->T1);
-  scratch_release(ctx, op->T3);
-  scratch_release(ctx, op->L32);
-  scratch_release(ctx, op->R32);
-  scratch_release(ctx, op->Lhi);
-  scratch_release(ctx, op->Llo);
-  scratch_release(ctx, op->Rhi);
-  scratch_release(ctx, op->Rlo);
-  delete op;
+  heff_release(ctx, op);
   API_END(ctx)
 }
 

@@ -114,6 +114,33 @@ def test_heff2_fp32(ctx, cl, cr):
     assert rel(got, chain.heff2_apply(L, W1, W2, R, th)) < 2e-5
 
 
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 2e-4), ("bf16", 2e-2)])
+@pytest.mark.parametrize("cl,cr,d,D", [(1, 1, 2, 1), (16, 16, 2, 5), (37, 70, 2, 5), (130, 129, 3, 5),
+                                       (256, 256, 2, 5)])
+def test_heff2_bf16_forms(ctx, prec, tol, cl, cr, d, D):
+    """BF16 MFMA forms: bf16x3 (hi/lo split operands, three products, FP32
+    accumulation) carries ~2^-16 relative error per product; single bf16 ~2^-8."""
+    import networksolvers_amd as ns
+    L, W1, W2, R, th = _heff_case(cl, cr, d, D, 19)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W1), ctx.upload(W2), ctx.upload(R), prec)
+    want = chain.heff2_apply(L, W1, W2, R, th)
+    x = ctx.upload(th)
+    for _ in range(2):   # the operator's workspace is reused across applies
+        got = H(x).numpy()
+        assert rel(got, want) < tol
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16"])
+def test_heff1_bf16_forms(ctx, prec):
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(23)
+    L, R = rng.standard_normal((40, 5, 40)), rng.standard_normal((33, 5, 33))
+    W = rng.standard_normal((5, 5, 2, 2))
+    x = rng.standard_normal((40, 2, 33))
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W), None, ctx.upload(R), prec)
+    assert rel(H(ctx.upload(x)).numpy(), chain.heff1_apply(L, W, R, x)) < (2e-4 if prec == "bf16x3" else 2e-2)
+
+
 def _physical_heff(N, chi, seed, lo):
     W = models.xxz_mpo(N, Jz=1.0)
     mps = chain.random_mps(N, 2, chi, seed=seed)
