@@ -30,8 +30,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=2, help="timed sweeps at chi = maxdim")
-    p.add_argument("--warmup", type=int, default=4, help="untimed ramp sweeps (maxdim 64, 256, 1024, ...)")
+    p.add_argument("--steps", type=int, default=3, help="timed sweeps at chi = maxdim")
+    p.add_argument("--warmup", type=int, default=8, help="untimed ramp sweeps (maxdim 64, 256, 1024, ...)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run one 256 MiB device-to-device tensor copy first (counter calibration under rocprofv3)")
     p.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3", "bf16"])
     p.add_argument("--nsites", type=int, default=100)
     p.add_argument("--maxdim", type=int, default=1024)
@@ -74,6 +76,12 @@ def gpu_sweeps(args):
     import networksolvers_amd as ns
     from networksolvers_amd import models
     ctx = ns.Context(0)
+    if args.calibrate_copy:
+        import numpy as np
+        big = ctx.empty((1 << 25,), np.float64)        # 256 MiB
+        big.copy().free()
+        ctx.sync()
+        big.free()
     N = args.nsites
     H = ns.TTNO(ctx, models.xxz_mpo(N, Jz=args.delta))
     psi = ns.TTN.from_arrays(ctx, models.random_mps(N, 2, 8, seed=args.seed))
