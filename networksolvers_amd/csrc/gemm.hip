@@ -184,12 +184,16 @@ void launch_tile(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T>& a) {
 
 template <typename T, int ROLE>
 void launch_role(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T>& a) {
-  // 128x128 tiles when they fill the chip (>= 2 per CU), else 64x64
+  // the largest tile that still gives every CU work: 128x128 when that makes >= 2
+  // workgroups per CU, 64x64 when >= 1 per CU, else 32x32 (small bond dimensions)
   const int64_t big = ((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
+  const int64_t mid = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
   if (big >= 512)
     launch_tile<T, 128, 128, ROLE>(ctx, d, a);
-  else
+  else if (mid >= 256)
     launch_tile<T, 64, 64, ROLE>(ctx, d, a);
+  else
+    launch_tile<T, 32, 32, ROLE>(ctx, d, a);
 }
 
 template <typename T>

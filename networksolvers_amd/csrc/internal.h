@@ -184,6 +184,16 @@ void vec_f32_to_f64(nsmi_ctx* ctx, double* y, const float* x, int64_t n);
 void vec_split_bf16(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const double* x, int64_t n);
 void vec_split_bf16_f32(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const float* x, int64_t n);
 void vec_check_finite(nsmi_ctx* ctx, const double* x, int64_t n, int slot);
+// complex (interleaved) vectors, n complex elements
+void vec_zscale(nsmi_ctx* ctx, double* x, int64_t n, double ar, double ai);
+void vec_zaxpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double ar, double ai);
+// slots [slot, slot+1] <- sum conj(x) y
+void vec_zdot_dev(nsmi_ctx* ctx, const double* x, const double* y, int64_t n, int slot);
+void vec_zlincomb(nsmi_ctx* ctx, double* out, const double* const* V, const double* cr, const double* ci, int k,
+                  int64_t n);
+// complex GEMM (zgemm.hip): C = op(A) op(B), interleaved complex, optional conjugation
+void gemm_c128(nsmi_ctx* ctx, const GemmDesc& d, const double* A, bool conjA, const double* B, bool conjB,
+               double* C, int role);
 
 // ---- MPO mixing (region.hip) --------------------------------------------
 // Y[o][p][i] = sum_q Mx[p][q] X[o][q][i]

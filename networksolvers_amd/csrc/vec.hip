@@ -138,7 +138,101 @@ __global__ void k_nonfinite_partial(const double* __restrict__ x, int64_t n, dou
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// ---- complex (interleaved re, im) vectors; n counts complex elements --------
+__global__ void k_zscale(double* __restrict__ x, int64_t n, double ar, double ai) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    x[2 * i] = ar * xr - ai * xi;
+    x[2 * i + 1] = ar * xi + ai * xr;
+  }
+}
+
+__global__ void k_zaxpy(double* __restrict__ y, const double* __restrict__ x, int64_t n, double ar, double ai) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    y[2 * i] += ar * xr - ai * xi;
+    y[2 * i + 1] += ar * xi + ai * xr;
+  }
+}
+
+// partials[b] = sum conj(x) y  (re), partials[kPartials + b] (im)
+__global__ void k_zdot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                               double* __restrict__ partials) {
+  double re = 0.0, im = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+    re += xr * yr + xi * yi;
+    im += xr * yi - xi * yr;
+  }
+  const double sr = block_sum(re);
+  const double si = block_sum(im);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = sr;
+    partials[kPartials + blockIdx.x] = si;
+  }
+}
+
+struct ZLinComb {
+  const double* v[32];
+  double cr[32], ci[32];
+};
+
+__global__ void k_zlincomb(double* __restrict__ out, ZLinComb lc, int k, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int j = 0; j < k; ++j) {
+      const double vr = lc.v[j][2 * i], vi = lc.v[j][2 * i + 1];
+      sr += lc.cr[j] * vr - lc.ci[j] * vi;
+      si += lc.cr[j] * vi + lc.ci[j] * vr;
+    }
+    out[2 * i] = sr;
+    out[2 * i + 1] = si;
+  }
+}
+
 }  // namespace
+
+void vec_zscale(nsmi_ctx* ctx, double* x, int64_t n, double ar, double ai) {
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 32.0 * n);
+  hipLaunchKernelGGL(k_zscale, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, x, n, ar, ai);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_zaxpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double ar, double ai) {
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 48.0 * n);
+  hipLaunchKernelGGL(k_zaxpy, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, ar, ai);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_zdot_dev(nsmi_ctx* ctx, const double* x, const double* y, int64_t n, int slot) {
+  require(slot >= 0 && slot + 1 < 64, "slot out of range");
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 32.0 * n);
+  int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, x, y, n, ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_partials, g, ctx->d_scal + slot);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, ctx->d_partials + kPartials, g,
+                     ctx->d_scal + slot + 1);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_zlincomb(nsmi_ctx* ctx, double* out, const double* const* V, const double* cr, const double* ci, int k,
+                  int64_t n) {
+  require(k >= 1 && k <= 32, "zlincomb: 1..32 vectors");
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 16.0 * (k + 1) * n);
+  ZLinComb lc{};
+  for (int j = 0; j < k; ++j) {
+    lc.v[j] = V[j];
+    lc.cr[j] = cr[j];
+    lc.ci[j] = ci[j];
+  }
+  hipLaunchKernelGGL(k_zlincomb, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, out, lc, k, n);
+  NSMI_LAUNCH_CHECK();
+}
 
 void vec_scale(nsmi_ctx* ctx, double* x, int64_t n, double a) {
   if (n <= 0) return;

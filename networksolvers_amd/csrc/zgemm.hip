@@ -1,0 +1,191 @@
+// Complex FP64 GEMM on v_mfma_f64_16x16x4 for the TDVP path (ComplexF64 states).
+//
+// Complex tensors are interleaved (re, im) doubles.  Tiles are staged into LDS as
+// separate real and imaginary planes (conjugation applied at staging), and each
+// 16x16 output block takes four real MFMAs:
+//   re += ar.br - ai.bi      im += ar.bi + ai.br
+// Row-major, op(A) M x K (A stored K x M when transA), op(B) K x N (stored N x K
+// when transB); C = op(A) op(B) (no scaling).  Output rows of the f64 MFMA are
+// interleaved across lane groups: item r of lane l is row l/16 + 4r.
+#include "internal.h"
+
+namespace nsmi {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+struct ZArgs {
+  const double* A;
+  const double* B;
+  double* C;
+  int64_t M, N, K, lda, ldb, ldc;  // leading dimensions in complex elements
+  int conjA, conjB;
+  int tiles_m, tiles_n;
+};
+
+__device__ inline int zxcd_remap(int wg, int nwg) {
+  constexpr int kXcd = 8;
+  if (nwg < 2 * kXcd || (nwg % kXcd) != 0) return wg;
+  return (wg % kXcd) * (nwg / kXcd) + wg / kXcd;
+}
+
+template <int BM, int BN, bool TA, bool TB, int ROLE>
+__global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
+  constexpr int BK = 8, PAD = 4;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
+  __shared__ double sAr[2][BK][BM + PAD], sAi[2][BK][BM + PAD];
+  __shared__ double sBr[2][BK][BN + PAD], sBi[2][BK][BN + PAD];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tile = zxcd_remap(blockIdx.x, g.tiles_m * g.tiles_n);
+  const int tm = tile / g.tiles_n, tn = tile % g.tiles_n;
+  const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
+  const double sa = g.conjA ? -1.0 : 1.0, sb = g.conjB ? -1.0 : 1.0;
+
+  d2 ra[EA > 0 ? EA : 1], rb[EB > 0 ? EB : 1];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      const int e = tid * EA + j;
+      const int mm = TA ? (e % BM) : (e / BK);
+      const int kk = TA ? (e / BM) : (e % BK);
+      const int64_t gm = m0 + mm, gk = k0 + kk;
+      d2 v = {0.0, 0.0};
+      if (gm < g.M && gk < g.K)
+        v = *reinterpret_cast<const d2*>(g.A + 2 * (TA ? gk * g.lda + gm : gm * g.lda + gk));
+      ra[j] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      const int e = tid * EB + j;
+      const int nn = TB ? (e / BK) : (e % BN);
+      const int kk = TB ? (e % BK) : (e / BN);
+      const int64_t gn = n0 + nn, gk = k0 + kk;
+      d2 v = {0.0, 0.0};
+      if (gn < g.N && gk < g.K)
+        v = *reinterpret_cast<const d2*>(g.B + 2 * (TB ? gn * g.ldb + gk : gk * g.ldb + gn));
+      rb[j] = v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < EA; ++j) {
+      const int e = tid * EA + j;
+      const int mm = TA ? (e % BM) : (e / BK);
+      const int kk = TA ? (e / BM) : (e % BK);
+      sAr[buf][kk][mm] = ra[j].x;
+      sAi[buf][kk][mm] = sa * ra[j].y;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
+      const int e = tid * EB + j;
+      const int nn = TB ? (e / BK) : (e % BN);
+      const int kk = TB ? (e % BK) : (e / BN);
+      sBr[buf][kk][nn] = rb[j].x;
+      sBi[buf][kk][nn] = sb * rb[j].y;
+    }
+  };
+
+  d4 acr[FM][FN], aci[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acr[i][j] = d4{0, 0, 0, 0};
+      aci[i][j] = d4{0, 0, 0, 0};
+    }
+  const int64_t nk = (g.K + BK - 1) / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  const int kl = lane >> 4, ml = lane & 15;
+  for (int64_t kt = 0; kt < nk; ++kt) {
+    const int buf = int(kt & 1);
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      double ar[FM], ai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        ar[i] = sAr[buf][ks * 4 + kl][wm * WTM + i * 16 + ml];
+        ai[i] = sAi[buf][ks * 4 + kl][wm * WTM + i * 16 + ml];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        br[j] = sBr[buf][ks * 4 + kl][wn * WTN + j * 16 + ml];
+        bi[j] = sBi[buf][ks * 4 + kl][wn * WTN + j * 16 + ml];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          acr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acr[i][j], 0, 0, 0);
+          acr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], acr[i][j], 0, 0, 0);
+          aci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], aci[i][j], 0, 0, 0);
+          aci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], aci[i][j], 0, 0, 0);
+        }
+    }
+    if (kt + 1 < nk) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * WTM + i * 16 + kl + 4 * r;
+        const int64_t gn = n0 + wn * WTN + j * 16 + ml;
+        if (gm < g.M && gn < g.N) {
+          d2 v = {acr[i][j][r], aci[i][j][r]};
+          *reinterpret_cast<d2*>(g.C + 2 * (gm * g.ldc + gn)) = v;
+        }
+      }
+}
+
+template <int BM, int BN, int ROLE>
+void zlaunch(nsmi_ctx* ctx, const GemmDesc& d, ZArgs& a) {
+  a.tiles_m = int((d.M + BM - 1) / BM);
+  a.tiles_n = int((d.N + BN - 1) / BN);
+  dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n));
+  if (!d.transA && !d.transB)
+    hipLaunchKernelGGL((k_zgemm<BM, BN, false, false, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+  else if (!d.transA && d.transB)
+    hipLaunchKernelGGL((k_zgemm<BM, BN, false, true, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+  else if (d.transA && !d.transB)
+    hipLaunchKernelGGL((k_zgemm<BM, BN, true, false, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+  else
+    hipLaunchKernelGGL((k_zgemm<BM, BN, true, true, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+  NSMI_LAUNCH_CHECK();
+}
+
+template <int ROLE>
+void zrole(nsmi_ctx* ctx, const GemmDesc& d, ZArgs& a) {
+  const int64_t mid = ((d.M + 63) / 64) * ((d.N + 63) / 64);
+  if (mid >= 256)
+    zlaunch<64, 64, ROLE>(ctx, d, a);
+  else
+    zlaunch<32, 32, ROLE>(ctx, d, a);
+}
+
+}  // namespace
+
+void gemm_c128(nsmi_ctx* ctx, const GemmDesc& d, const double* A, bool conjA, const double* B, bool conjB,
+               double* C, int role) {
+  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch == 1, "zgemm: bad sizes (no batching)");
+  if (d.M == 0 || d.N == 0) return;
+  ZArgs a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, conjA ? 1 : 0, conjB ? 1 : 0, 0, 0};
+  switch (role) {
+    case kRoleHeff1: zrole<kRoleHeff1>(ctx, d, a); break;
+    case kRoleHeff2: zrole<kRoleHeff2>(ctx, d, a); break;
+    case kRoleEnv: zrole<kRoleEnv>(ctx, d, a); break;
+    case kRoleTheta: zrole<kRoleTheta>(ctx, d, a); break;
+    default: zrole<kRoleFact>(ctx, d, a); break;
+  }
+}
+
+}  // namespace nsmi

@@ -1,0 +1,200 @@
+"""Continuous-batching request loop for independent ground-state solves on one GPU.
+
+A request is a 2-site DMRG solve of an XXZ chain (N, Delta, maxdim, sweeps,
+cutoff, seed, precision).  The server keeps ``slots`` worker threads, each
+driving its own device context (own HIP stream and pool), so the region steps
+of all admitted solves run concurrently on the device and the latency-bound
+phases of one solve (the dense eigensolver, host round trips) overlap with the
+work of the others.  States and environments stay resident in HBM for the life
+of a solve.
+
+Admission is continuous and by HBM footprint: a queued request (in arrival
+order) is admitted as soon as a slot is free and its footprint estimate fits in
+the remaining budget; a solve that outgrows its estimate fails with
+NSMI_ERR_OOM inside its own pool instead of starving the others.
+
+The reference has no serving layer; its control-flow model is sweep_solve.jl:12-40
+(the same dmrg/eigsolve entry point is called per request).
+"""
+from __future__ import annotations
+
+import heapq
+import math
+import queue
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import models
+from .device import Context
+from .network import ProjTTN, TTN, TTNO, expect_mpo
+from .sweep import EigsolveProblem, eigsolve_problem
+
+
+@dataclass
+class SolveRequest:
+    rid: int
+    N: int
+    delta: float
+    maxdim: int
+    nsweeps: int = 4
+    cutoff: float = 1e-10
+    seed: int = 0
+    precision: str = "fp32"
+    chi0: int = 4
+    arrival: float = 0.0          # seconds after the stream start
+
+
+@dataclass
+class SolveResult:
+    rid: int
+    energy: float                 # FP64 <psi|H|psi> of the final state
+    ritz: float                   # last region's Ritz value (the reference's eigenvalue)
+    maxlinkdim: int
+    arrival: float
+    admitted: float
+    finished: float
+    error: str | None = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def footprint_bytes(req: SolveRequest, d: int = 2, D: int = 5) -> int:
+    """Upper estimate of a solve's device memory: MPS + environments at
+    bond dimension maxdim, one H_eff workspace, the Lanczos basis and the
+    factorisation scratch, plus a fixed pool overhead."""
+    chi = req.maxdim
+    mps = req.N * chi * d * chi * 8
+    envs = (req.N + 1) * chi * D * chi * 8
+    heff = 2 * chi * D * d * d * chi * 8
+    krylov = 6 * chi * d * d * chi * 8
+    fact = 6 * (chi * d) ** 2 * 8
+    return int(1.5 * (mps + envs + heff + krylov + fact)) + (64 << 20)
+
+
+def run_solve(ctx: Context, req: SolveRequest) -> tuple[float, float, int]:
+    """One request on a context: seeded random start, nsweeps of 2-site DMRG
+    with maxdim/cutoff, FP64 energy of the final state."""
+    W = models.xxz_mpo(req.N, Jz=req.delta)
+    H = TTNO(ctx, W)
+    psi = TTN.from_arrays(ctx, models.random_mps(req.N, 2, req.chi0, seed=req.seed))
+    prob = EigsolveProblem(state=psi, operator=ProjTTN(H))
+    ritz, psi = eigsolve_problem(prob, nsweeps=req.nsweeps, nsites=2,
+                                 updater_kwargs=dict(precision=req.precision),
+                                 inserter_kwargs=dict(trunc=dict(cutoff=req.cutoff, maxdim=req.maxdim)))
+    e = expect_mpo(psi, H)
+    return e, ritz, psi.maxlinkdim()
+
+
+class Server:
+    """Slots x contexts with HBM-footprint admission (see module docstring)."""
+
+    def __init__(self, slots: int = 8, hbm_budget: int = 200 << 30, device: int = 0):
+        self.slots = slots
+        self.budget = hbm_budget
+        self.device = device
+        self.contexts = [Context(device) for _ in range(slots)]
+        self._lock = threading.Condition()
+        self._free_slots = list(range(slots))
+        self._in_use_bytes = 0
+        self.peak_bytes = 0
+        self.peak_concurrency = 0
+        self._active = 0
+
+    def close(self):
+        for c in self.contexts:
+            c.close()
+
+    def _worker(self, slot, req, fp, t0, results):
+        ctx = self.contexts[slot]
+        admitted = time.perf_counter() - t0
+        try:
+            e, ritz, mld = run_solve(ctx, req)
+            err = None
+        except Exception as exc:  # a failed solve is reported, the server keeps going
+            e, ritz, mld, err = math.nan, math.nan, 0, f"{type(exc).__name__}: {exc}"
+        finished = time.perf_counter() - t0
+        ctx.trim()
+        results.append(SolveResult(req.rid, e, ritz, mld, req.arrival, admitted, finished, err))
+        with self._lock:
+            self._free_slots.append(slot)
+            self._in_use_bytes -= fp
+            self._active -= 1
+            self._lock.notify_all()
+
+    def run(self, requests, realtime: bool = True):
+        """Serve ``requests`` (sorted by arrival).  With ``realtime`` the arrival
+        times are honoured (Poisson stream); otherwise all are queued at t=0."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        results: list[SolveResult] = []
+        threads = []
+        t0 = time.perf_counter()
+        pending = []  # heap of (arrival, rid, req)
+        i = 0
+        while i < len(reqs) or pending:
+            now = time.perf_counter() - t0
+            while i < len(reqs) and (not realtime or reqs[i].arrival <= now):
+                heapq.heappush(pending, (reqs[i].arrival, reqs[i].rid, reqs[i]))
+                i += 1
+            admitted_any = False
+            with self._lock:
+                while pending and self._free_slots:
+                    req = pending[0][2]
+                    fp = footprint_bytes(req)
+                    if fp > self.budget:
+                        heapq.heappop(pending)
+                        results.append(SolveResult(req.rid, math.nan, math.nan, 0, req.arrival, now, now,
+                                                   "rejected: footprint above the HBM budget"))
+                        continue
+                    if self._in_use_bytes + fp > self.budget:
+                        break  # wait for memory to free up (arrival order is kept)
+                    heapq.heappop(pending)
+                    slot = self._free_slots.pop()
+                    self._in_use_bytes += fp
+                    self.peak_bytes = max(self.peak_bytes, self._in_use_bytes)
+                    self._active += 1
+                    self.peak_concurrency = max(self.peak_concurrency, self._active)
+                    th = threading.Thread(target=self._worker, args=(slot, req, fp, t0, results), daemon=True)
+                    th.start()
+                    threads.append(th)
+                    admitted_any = True
+                if i >= len(reqs) and not pending:
+                    break
+                if not admitted_any:
+                    # sleep until the next arrival or until a solve frees a slot / memory
+                    timeout = 0.005
+                    if i < len(reqs):
+                        timeout = min(timeout, max(0.0, reqs[i].arrival - (time.perf_counter() - t0)))
+                    self._lock.wait(timeout=timeout)
+        for th in threads:
+            th.join()
+        wall = time.perf_counter() - t0
+        return sorted(results, key=lambda r: r.rid), wall
+
+
+def make_stream(n: int, rate: float, seed: int = 0, *, n_range=(16, 128), delta_range=(0.0, 2.0),
+                maxdims=(64, 128, 256), nsweeps=4, cutoff=1e-10, precision="fp32"):
+    """The serving workload: Poisson arrivals at ``rate`` per second; each request
+    an XXZ chain with N ~ U{n_range}, Delta ~ U[delta_range], maxdim ~ U{maxdims}."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        out.append(SolveRequest(rid=rid, N=int(rng.integers(n_range[0], n_range[1] + 1)),
+                                delta=float(rng.uniform(*delta_range)), maxdim=int(rng.choice(maxdims)),
+                                nsweeps=nsweeps, cutoff=cutoff, seed=int(rng.integers(1 << 31)),
+                                precision=precision, arrival=t))
+    return out
+
+
+def latency_stats(results):
+    lat = np.array([r.latency for r in results if r.error is None])
+    if lat.size == 0:
+        return {"p50_s": None, "p99_s": None, "mean_s": None}
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean())}

@@ -233,6 +233,7 @@ class RegionLog:
     kept: int
     truncerr: float
     route: str
+    seconds: float = 0.0   # wall time of this region step (after any initial gauge walk)
 
 
 @dataclass
@@ -247,7 +248,7 @@ class DMRGResult:
 
 def dmrg_chain(W, mps0, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1,
                krylovdim=3, maxiter=1, tol=1e-14, root_vertex=None,
-               which_decomp=None, normalize=False, max_regions=None):
+               which_decomp=None, normalize=False, max_regions=None, max_seconds=None):
     """2-site DMRG on a chain, restating the region loop of
     sweep_solve.jl:12-40 -> iterators.jl:81-100 with
       plan      euler_sweep(nsites=2) from root_vertex (euler_plans.jl:4-13),
@@ -258,7 +259,7 @@ def dmrg_chain(W, mps0, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1,
                 (inserter.jl:3-33),
     with per-sweep truncation parameters (truncation_parameters.jl:7-14).
     Returns the last region's Ritz value as the energy (eigsolve.jl:23,66).
-    ``max_regions`` stops early (bounded timing samples only).
+    ``max_regions`` / ``max_seconds`` stop early (bounded timing samples only).
     """
     N = len(W)
     if root_vertex is None:
@@ -280,6 +281,7 @@ def dmrg_chain(W, mps0, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1,
                 mps = orthogonalize_chain(mps, lo, hi)
                 for i in range(N):
                     envs.site_changed(i)
+            t_region = time.perf_counter()
             th = theta2(mps[lo], mps[hi])
             L = envs.get(mps, lo - 1, lo)
             R = envs.get(mps, hi + 1, hi)
@@ -305,9 +307,11 @@ def dmrg_chain(W, mps0, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1,
             envs.site_changed(lo)
             envs.site_changed(hi)
             ortho = v
-            res.log.append(RegionLog(sweep, (u, v), energy, info.kept, info.truncerr, info.route))
+            res.log.append(RegionLog(sweep, (u, v), energy, info.kept, info.truncerr, info.route,
+                                     time.perf_counter() - t_region))
             nreg += 1
-            if max_regions is not None and nreg >= max_regions:
+            over_time = max_seconds is not None and time.perf_counter() - t0 > max_seconds
+            if (max_regions is not None and nreg >= max_regions) or over_time:
                 res.energy, res.mps = energy, mps
                 res.seconds = time.perf_counter() - t0
                 return res

@@ -228,6 +228,30 @@ def test_factorize(ctx, route, trailing, shape, cutoff, maxdim):
     np.testing.assert_allclose(Cg[good], Co[good], atol=1e-9)
 
 
+@pytest.mark.parametrize("m,n,maxdim", [(2, 5, None), (3, 3, None), (600, 620, 300), (700, 700, 350),
+                                         (150, 150, None)])
+def test_factorize_eigensolver_paths(ctx, m, n, maxdim):
+    """Gram sizes on both sides of the single-workgroup tridiagonalisation limit
+    (640; rocSOLVER dsyevd above it), with a graded spectrum."""
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(m + n)
+    r = min(m, n)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    s = np.exp(-np.arange(r) / 25.0)
+    M = (U0 * s) @ V0.T
+    th = M.reshape(m, 1, 1, n)
+    U, C, info = ns.factorize(ctx.upload(th), 2, False, cutoff=1e-9, maxdim=maxdim, which_decomp="eigen")
+    Uo, Co, io = linalg.factorize(M, cutoff=1e-9, maxdim=maxdim, which_decomp="eigen")
+    assert info.kept == io.kept
+    assert abs(info.truncerr - io.truncerr) < 1e-13 + 1e-8 * io.truncerr
+    Ug, Cg = U.numpy().reshape(m, -1), C.numpy().reshape(-1, n)
+    np.testing.assert_allclose(Ug.T @ Ug, np.eye(info.kept), atol=1e-12)
+    np.testing.assert_allclose(Ug @ Cg, Uo @ Co, atol=1e-12)
+    good = io.spectrum[:info.kept] >= 1e-6 * io.spectrum[0]
+    np.testing.assert_allclose(Ug[:, good], Uo[:, good], atol=1e-9)
+
+
 @pytest.mark.parametrize("trailing", [False, True])
 @pytest.mark.parametrize("shape", [(3, 2, 5), (8, 2, 3), (16, 2, 16), (1, 2, 4), (40, 3, 100)])
 def test_qr(ctx, trailing, shape):

@@ -1,0 +1,90 @@
+"""Serving loop: continuous admission into concurrent device contexts.
+
+Bar: every request's FP64 energy and final bond dimension are the same when it
+is solved alone and when it is solved in a concurrent mix (the kernels are
+deterministic per context, so the tolerance is 1e-12 relative); admission never
+exceeds the HBM budget; results match the oracle's DMRG for a sample.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs():
+    from networksolvers_amd.serving import SolveRequest
+    spec = [(12, 0.3, 16), (20, 1.0, 32), (16, 1.7, 24), (24, 0.8, 32), (14, 1.2, 16), (18, 0.0, 32)]
+    return [SolveRequest(rid=i, N=n, delta=dl, maxdim=m, nsweeps=3, cutoff=1e-10, seed=100 + i,
+                         precision="fp32", arrival=0.0) for i, (n, dl, m) in enumerate(spec)]
+
+
+def test_same_result_alone_and_in_the_mix():
+    from networksolvers_amd.serving import Server
+    reqs = _reqs()
+    alone = {}
+    srv = Server(slots=1)
+    try:
+        for r in reqs:
+            res, _ = srv.run([r], realtime=False)
+            assert res[0].error is None, res[0].error
+            alone[r.rid] = res[0]
+    finally:
+        srv.close()
+    srv = Server(slots=len(reqs))
+    try:
+        mix, _ = srv.run(reqs, realtime=False)
+        assert srv.peak_concurrency >= 2
+    finally:
+        srv.close()
+    for m in mix:
+        a = alone[m.rid]
+        assert m.error is None, m.error
+        assert abs(m.energy - a.energy) <= 1e-12 * abs(a.energy)
+        assert m.maxlinkdim == a.maxlinkdim
+
+
+def test_matches_oracle_dmrg():
+    from networksolvers_amd.serving import Server
+    from oracle import chain, models
+    reqs = _reqs()[:2]
+    for r in reqs:
+        r.precision = "fp64"
+    srv = Server(slots=2)
+    try:
+        res, _ = srv.run(reqs, realtime=False)
+    finally:
+        srv.close()
+    for r, got in zip(reqs, res):
+        W = models.xxz_mpo(r.N, Jz=r.delta)
+        ref = chain.dmrg_chain(W, chain.random_mps(r.N, 2, r.chi0, seed=r.seed), nsweeps=r.nsweeps,
+                               maxdim=r.maxdim, cutoff=r.cutoff)
+        e_ref = chain.mpo_expectation(W, ref.mps) / chain.mps_inner(ref.mps, ref.mps)
+        assert abs(got.energy - e_ref) < 1e-10 * abs(e_ref)
+
+
+def test_admission_respects_hbm_budget():
+    from networksolvers_amd.serving import Server, footprint_bytes
+    reqs = _reqs()
+    budget = 2 * max(footprint_bytes(r) for r in reqs)
+    srv = Server(slots=6, hbm_budget=budget)
+    try:
+        res, _ = srv.run(reqs, realtime=False)
+        assert srv.peak_bytes <= budget
+        assert srv.peak_concurrency <= 2
+    finally:
+        srv.close()
+    assert all(r.error is None for r in res)
+
+
+def test_poisson_stream_latency_report():
+    from networksolvers_amd.serving import Server, latency_stats, make_stream
+    reqs = make_stream(10, rate=20.0, seed=3, n_range=(10, 20), maxdims=(8, 16), nsweeps=2)
+    srv = Server(slots=4)
+    try:
+        res, wall = srv.run(reqs, realtime=True)
+    finally:
+        srv.close()
+    st = latency_stats(res)
+    assert all(r.error is None for r in res)
+    assert st["p50_s"] is not None and st["p99_s"] >= st["p50_s"] > 0
+    assert all(r.admitted >= r.arrival - 1e-3 for r in res)
