@@ -161,6 +161,15 @@ int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, doubl
 int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_lanczos_opts* opts,
                      double* eigval, nsmi_solver_info* info);
 
+/* x <- exp(tau H) x for the Hermitian operator (exponentiate_solver,
+ * local_solvers/exponentiate.jl:3-29; tau = -i * weight * dt in TDVP).
+ * opts: krylovdim (<= 32, reference default 30), maxiter (sub-steps, 100),
+ * tol (error per unit time, 1e-12), eta (re-orthogonalisation threshold).
+ * info: numops = H applies, krylov_dim = largest basis used, normres = last
+ * error estimate.  A complex tau needs a complex (C128) state. */
+int nsmi_krylov_expm(nsmi_ctx* ctx, nsmi_heff* op, double tau_re, double tau_im, nsmi_tensor* x,
+                     const nsmi_lanczos_opts* opts, nsmi_solver_info* info);
+
 /* Truncated split of theta (ITensors factorize, ortho="left"; inserter.jl:23).
  * rows = the first n_row_dims dims of theta when rows_trailing == 0, else the
  * remaining trailing dims.  The isometry U carries the row dims:

@@ -83,6 +83,8 @@ SIGNATURES = {
     "nsmi_heff_rayleigh": (C.c_int, [_VP, _VP, _VP, C.POINTER(C.c_double)]),
     "nsmi_lanczos_eig": (C.c_int, [_VP, _VP, _VP, C.POINTER(LanczosOpts), C.POINTER(C.c_double),
                                    C.POINTER(SolverInfo)]),
+    "nsmi_krylov_expm": (C.c_int, [_VP, _VP, C.c_double, C.c_double, _VP, C.POINTER(LanczosOpts),
+                                   C.POINTER(SolverInfo)]),
     "nsmi_factorize": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.POINTER(Trunc), _PP, _PP, C.POINTER(TruncInfo)]),
     "nsmi_qr": (C.c_int, [_VP, _VP, C.c_int, C.c_int, _PP, _PP]),
     "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),

@@ -323,9 +323,12 @@ int nsmi_tensor_reshape(nsmi_ctx* ctx, nsmi_tensor* t, int rank, const int64_t* 
 int nsmi_tensor_scale(nsmi_ctx* ctx, nsmi_tensor* t, double re, double im) {
   if (!ctx || !t) return NSMI_ERR_INVALID;
   API_BEGIN
-  if (t->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "scale: only F64 tensors");
-  require(im == 0.0, "scale: imaginary factor on a real tensor");
-  vec_scale(ctx, static_cast<double*>(t->data), t->numel, re);
+  if (t->dtype == NSMI_C128) {
+    vec_zscale(ctx, static_cast<double*>(t->data), t->numel, re, im);
+  } else {
+    require(im == 0.0, "scale: imaginary factor on a real tensor");
+    vec_scale(ctx, static_cast<double*>(t->data), t->numel, re);
+  }
   API_END(ctx)
 }
 
@@ -333,18 +336,21 @@ int nsmi_tensor_axpy(nsmi_ctx* ctx, double re, double im, const nsmi_tensor* x, 
   if (!ctx || !x || !y) return NSMI_ERR_INVALID;
   API_BEGIN
   require(x->numel == y->numel && x->dtype == y->dtype, "axpy: size/dtype mismatch");
-  if (x->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "axpy: only F64 tensors");
-  require(im == 0.0, "axpy: imaginary factor on a real tensor");
-  vec_axpy(ctx, static_cast<double*>(y->data), static_cast<const double*>(x->data), x->numel, re);
+  if (x->dtype == NSMI_C128) {
+    vec_zaxpy(ctx, static_cast<double*>(y->data), static_cast<const double*>(x->data), x->numel, re, im);
+  } else {
+    require(im == 0.0, "axpy: imaginary factor on a real tensor");
+    vec_axpy(ctx, static_cast<double*>(y->data), static_cast<const double*>(x->data), x->numel, re);
+  }
   API_END(ctx)
 }
 
 int nsmi_tensor_norm(nsmi_ctx* ctx, const nsmi_tensor* t, double* out) {
   if (!ctx || !t || !out) return NSMI_ERR_INVALID;
   API_BEGIN
-  if (t->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "norm: only F64 tensors");
   const double* x = static_cast<const double*>(t->data);
-  vec_dot_dev(ctx, x, x, t->numel, 0);
+  const int64_t nd = t->dtype == NSMI_C128 ? 2 * t->numel : t->numel;
+  vec_dot_dev(ctx, x, x, nd, 0);
   double v;
   read_slots(ctx, 1, &v);
   *out = std::sqrt(v);
@@ -355,12 +361,16 @@ int nsmi_tensor_inner(nsmi_ctx* ctx, const nsmi_tensor* a, const nsmi_tensor* b,
   if (!ctx || !a || !b) return NSMI_ERR_INVALID;
   API_BEGIN
   require(a->numel == b->numel && a->dtype == b->dtype, "inner: size/dtype mismatch");
-  if (a->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "inner: only F64 tensors");
-  vec_dot_dev(ctx, static_cast<const double*>(a->data), static_cast<const double*>(b->data), a->numel, 0);
-  double v;
-  read_slots(ctx, 1, &v);
-  if (re) *re = v;
-  if (im) *im = 0.0;
+  double v[2] = {0.0, 0.0};
+  if (a->dtype == NSMI_C128) {
+    vec_zdot_dev(ctx, static_cast<const double*>(a->data), static_cast<const double*>(b->data), a->numel, 0);
+    read_slots(ctx, 2, v);
+  } else {
+    vec_dot_dev(ctx, static_cast<const double*>(a->data), static_cast<const double*>(b->data), a->numel, 0);
+    read_slots(ctx, 1, v);
+  }
+  if (re) *re = v[0];
+  if (im) *im = v[1];
   API_END(ctx)
 }
 

@@ -232,7 +232,7 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
   API_BEGIN
   *Uout = nullptr;
   *Cout = nullptr;
-  if (theta->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "factorize: only F64 tensors are served");
+  require(theta->dtype == NSMI_F64 || theta->dtype == NSMI_C128, "factorize: unknown element type");
   require(n_row_dims >= 1 && n_row_dims < theta->rank, "factorize: n_row_dims must split theta's dims");
   int64_t m = 1, nc = 1;  // M = theta viewed as m x nc (leading dims x trailing dims)
   for (int i = 0; i < n_row_dims; ++i) m *= theta->dims[i];
@@ -247,6 +247,20 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
   int route = trunc->route;
   if (route == NSMI_ROUTE_AUTO) route = (trunc->cutoff <= 1e-12) ? NSMI_ROUTE_SVD : NSMI_ROUTE_EIGEN;
   require(route == NSMI_ROUTE_SVD || route == NSMI_ROUTE_EIGEN, "factorize: unknown route");
+  if (theta->dtype == NSMI_C128) {
+    int64_t kk = 0;
+    double te = 0.0, tot = 0.0;
+    zfactorize(ctx, theta, n_row_dims, rows_trailing, trunc, route, maxdim, mindim, Uout, Cout, &kk, &te, &tot,
+               truncate_spectrum);
+    if (info) {
+      info->kept = kk;
+      info->full = full;
+      info->truncerr = te;
+      info->total = tot;
+      info->route = route;
+    }
+    return NSMI_OK;
+  }
   const double* M = static_cast<const double*>(theta->data);
 
   std::vector<double> P;
@@ -421,8 +435,12 @@ int nsmi_qr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_traili
   API_BEGIN
   *Qout = nullptr;
   *Rout = nullptr;
-  if (A->dtype != NSMI_F64) fail(NSMI_ERR_UNSUPPORTED, "qr: only F64 tensors are served");
+  require(A->dtype == NSMI_F64 || A->dtype == NSMI_C128, "qr: unknown element type");
   require(n_row_dims >= 1 && n_row_dims < A->rank, "qr: n_row_dims must split A's dims");
+  if (A->dtype == NSMI_C128) {
+    zqr(ctx, A, n_row_dims, rows_trailing, Qout, Rout);
+    return NSMI_OK;
+  }
   int64_t m = 1, nc = 1;
   for (int i = 0; i < n_row_dims; ++i) m *= A->dims[i];
   for (int i = n_row_dims; i < A->rank; ++i) nc *= A->dims[i];

@@ -204,4 +204,17 @@ void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t 
 
 void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jacobi, ascending
 
+// complex factorize / QR (zfactorize.hip); the truncation rule is passed in
+void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing, const nsmi_trunc* trunc,
+                int route, int64_t maxdim, int64_t mindim, nsmi_tensor** Uout, nsmi_tensor** Cout, int64_t* kept,
+                double* truncerr, double* total,
+                int64_t (*truncate)(std::vector<double>&, int64_t, int64_t, double, double*, double*));
+void zqr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Qout,
+         nsmi_tensor** Rout);
+
+// H_eff operator access for the Krylov solvers (krylov.cpp)
+void heff_apply_dev(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y);
+int64_t heff_size(const nsmi_heff* op);
+bool heff_is_complex(const nsmi_heff* op);
+
 }  // namespace nsmi

@@ -1,0 +1,320 @@
+// Krylov solvers over a device H_eff: the Lanczos eigensolver (KrylovKit eigsolve)
+// and the Krylov exponentiator (KrylovKit exponentiate), for real (F64) and complex
+// (C128) vectors.  Vectors stay in HBM; scalars return to the host per reduction;
+// the projected tridiagonal problems are solved on the host (K <= 32).
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "internal.h"
+
+using namespace nsmi;
+using cplx = std::complex<double>;
+
+namespace {
+
+struct KOps {
+  nsmi_ctx* ctx;
+  int64_t n;  // elements (complex elements when cplx)
+  bool is_c;
+
+  cplx dot(const double* a, const double* b) const {  // sum conj(a) b
+    double r[2] = {0.0, 0.0};
+    if (is_c) {
+      vec_zdot_dev(ctx, a, b, n, 0);
+      read_slots(ctx, 2, r);
+      return {r[0], r[1]};
+    }
+    vec_dot_dev(ctx, a, b, n, 0);
+    read_slots(ctx, 1, r);
+    return {r[0], 0.0};
+  }
+  double norm(const double* a) const {
+    double r;
+    vec_dot_dev(ctx, a, a, is_c ? 2 * n : n, 0);
+    read_slots(ctx, 1, &r);
+    return std::sqrt(std::max(r, 0.0));
+  }
+  void axpy(double* y, const double* x, cplx c) const {  // y += c x
+    if (is_c)
+      vec_zaxpy(ctx, y, x, n, c.real(), c.imag());
+    else
+      vec_axpy(ctx, y, x, n, c.real());
+  }
+  void scale(double* x, double a) const { vec_scale(ctx, x, is_c ? 2 * n : n, a); }
+  void scaled_copy(double* y, const double* x, double a) const { vec_scaled_copy(ctx, y, x, is_c ? 2 * n : n, a); }
+  void lincomb(double* out, const std::vector<const double*>& V, const std::vector<cplx>& c) const {
+    const int k = int(V.size());
+    if (is_c) {
+      std::vector<double> cr(k), ci(k);
+      for (int j = 0; j < k; ++j) {
+        cr[j] = c[j].real();
+        ci[j] = c[j].imag();
+      }
+      vec_zlincomb(ctx, out, V.data(), cr.data(), ci.data(), k, n);
+    } else {
+      std::vector<double> cr(k);
+      for (int j = 0; j < k; ++j) cr[j] = c[j].real();
+      vec_lincomb(ctx, out, V.data(), cr.data(), k, n);
+    }
+  }
+  size_t bytes() const { return size_t(n) * (is_c ? 16 : 8); }
+  size_t doubles() const { return size_t(n) * (is_c ? 2 : 1); }
+};
+
+const double kEps = 2.220446049250313e-16;
+
+// Lanczos basis growth shared by both solvers: given V[0..k-1] and the residual r
+// with norm beta (r orthogonal to V up to rounding), appends v_k = r / beta, applies
+// H, and returns the new (alpha_k, beta_k) with r the new residual.  Three-term
+// recurrence, then modified Gram-Schmidt against the whole basis while the norm
+// drops below eta of its previous value (KrylovKit ModifiedGramSchmidtIR).
+void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r, double beta, double eta,
+                    double* alpha_out, double* beta_out) {
+  const size_t nd = ops.doubles();
+  double* vk = V + nd * k;
+  double* vprev = V + nd * (k - 1);
+  ops.scaled_copy(vk, r, 1.0 / beta);
+  heff_apply_dev(ops.ctx, op, vk, r);
+  ops.axpy(r, vprev, cplx(-beta, 0.0));
+  cplx c = ops.dot(vk, r);
+  double a = c.real();
+  ops.axpy(r, vk, -c);
+  double b = ops.norm(r);
+  double nold = std::sqrt(b * b + a * a + beta * beta);
+  while (kEps < b && b < eta * nold) {
+    nold = b;
+    cplx cl;
+    for (int j = 0; j <= k; ++j) {
+      const double* q = V + nd * j;
+      cl = ops.dot(q, r);
+      ops.axpy(r, q, -cl);
+    }
+    a += cl.real();
+    b = ops.norm(r);
+  }
+  *alpha_out = a;
+  *beta_out = b;
+}
+
+// first Lanczos vector from x0: V[0] = x0 / |x0|, r = H V[0] - alpha V[0], reorthogonalised
+void lanczos_start(const KOps& ops, nsmi_heff* op, const double* x0, double beta0, double* V, double* r,
+                   double eta, double* alpha_out, double* beta_out) {
+  heff_apply_dev(ops.ctx, op, x0, r);  // H x0
+  cplx c = ops.dot(x0, r);
+  double alpha = c.real() / (beta0 * beta0);
+  ops.scaled_copy(V, x0, 1.0 / beta0);
+  ops.scale(r, 1.0 / beta0);
+  double beta_old = ops.norm(r);
+  ops.axpy(r, V, cplx(-alpha, 0.0));
+  double beta = ops.norm(r);
+  while (kEps < beta && beta < eta * beta_old) {
+    beta_old = beta;
+    cplx d = ops.dot(V, r);
+    alpha += d.real();
+    ops.axpy(r, V, -d);
+    beta = ops.norm(r);
+  }
+  *alpha_out = alpha;
+  *beta_out = beta;
+}
+
+void tridiag_eigh(const std::vector<double>& alphas, const std::vector<double>& betas, std::vector<double>& w,
+                  std::vector<double>& S) {
+  const int k = int(alphas.size());
+  std::vector<double> T(size_t(k) * k, 0.0);
+  for (int i = 0; i < k; ++i) T[i * k + i] = alphas[i];
+  for (int i = 0; i + 1 < k; ++i) T[i * k + i + 1] = T[(i + 1) * k + i] = betas[i];
+  w.assign(k, 0.0);
+  S.assign(size_t(k) * k, 0.0);
+  small_sym_eigh(k, T.data(), w.data(), S.data());
+}
+
+cplx phi1(cplx z) {
+  if (std::abs(z) < 1e-5) return 1.0 + z * (0.5 + z / 6.0);
+  return (std::exp(z) - 1.0) / z;
+}
+
+}  // namespace
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                             \
+  }                                                               \
+  catch (const nsmi::Error& e) {                                  \
+    if (ctxp) (ctxp)->err = e.what();                             \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what(); \
+    return NSMI_ERR_INVALID;                                      \
+  }                                                               \
+  return NSMI_OK;
+
+extern "C" {
+
+// Restates KrylovKit eigsolve(f, x0, 1, :SR; ishermitian=true, krylovdim, maxiter=1,
+// tol, eager=false) with the ModifiedGramSchmidtIR(eta) orthogonaliser
+// (local_solvers/eigsolve.jl:3-29).
+int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_lanczos_opts* opts, double* eigval,
+                     nsmi_solver_info* info) {
+  if (!ctx || !op || !x || !eigval) return NSMI_ERR_INVALID;
+  API_BEGIN
+  const bool cp = heff_is_complex(op);
+  require(x->dtype == (cp ? NSMI_C128 : NSMI_F64), "lanczos: x element type does not match the operator");
+  require(x->numel == heff_size(op), "lanczos: x does not match the operator");
+  const int K = opts ? opts->krylovdim : 3;
+  const int maxiter = opts ? opts->maxiter : 1;
+  const double tol = opts ? opts->tol : 1e-14;
+  const double eta = opts ? opts->eta : 1.0 / std::sqrt(2.0);
+  require(K >= 1 && K <= 32, "lanczos: krylovdim must be in [1, 32]");
+  if (maxiter != 1) fail(NSMI_ERR_UNSUPPORTED, "lanczos: only maxiter = 1 is served");
+  KOps ops{ctx, x->numel, cp};
+  const size_t nd = ops.doubles();
+  Scratch basis(ctx, sizeof(double) * nd * (K + 1));
+  double* V = basis.as<double>();
+  double* r = V + nd * K;
+  double* xd = static_cast<double*>(x->data);
+  const double beta0 = ops.norm(xd);
+  require(beta0 > 0.0, "lanczos: initial vector should not have norm zero");
+  double alpha, beta;
+  lanczos_start(ops, op, xd, beta0, V, r, eta, &alpha, &beta);
+  int numops = 1;
+  std::vector<double> alphas{alpha}, betas;
+  int k = 1;
+  while (!(k == K || beta <= tol)) {
+    double a, b;
+    betas.push_back(beta);
+    lanczos_expand(ops, op, V, k, r, beta, eta, &a, &b);
+    ++numops;
+    alphas.push_back(a);
+    beta = b;
+    ++k;
+  }
+  std::vector<double> w, S;
+  tridiag_eigh(alphas, betas, w, S);
+  std::vector<cplx> y(k);
+  double ymax = 0.0;
+  for (int i = 0; i < k; ++i) ymax = std::max(ymax, std::fabs(S[i * k + 0]));
+  int i0 = 0;
+  while (i0 < k - 1 && !(std::fabs(S[i0 * k + 0]) > 1e-14 * ymax)) ++i0;
+  const double sg = S[i0 * k + 0] < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < k; ++i) y[i] = sg * S[i * k + 0];
+  std::vector<const double*> Vp(k);
+  for (int j = 0; j < k; ++j) Vp[j] = V + nd * j;
+  ops.lincomb(xd, Vp, y);
+  *eigval = w[0];
+  if (info) {
+    info->numops = numops;
+    info->krylov_dim = k;
+    info->normres = std::fabs(beta * y[k - 1].real());
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  API_END(ctx)
+}
+
+// exp(tau H) x for Hermitian H (exponentiate_solver, local_solvers/exponentiate.jl:3-29).
+// Lanczos basis from the current vector with the same recurrence and
+// re-orthogonalisation as the eigensolver; after every new basis vector (eager)
+// the projected exponential y = exp(z T_K) e1, z = sign(tau) dt, is formed from the
+// eigendecomposition of the K x K tridiagonal T_K and accepted when Saad's error
+// estimate beta0 * beta_K * dt * |[phi1(z T_K) e1]_K| <= tol * dt (tol per unit
+// time); at K = krylovdim the step shrinks by clamp(0.9 (tol dt / err)^(1/K), 0.1,
+// 0.9) until the estimate passes, the accepted step carries over to the next
+// sub-step, and sub-steps continue until the full time is covered (at most maxiter).
+int nsmi_krylov_expm(nsmi_ctx* ctx, nsmi_heff* op, double tau_re, double tau_im, nsmi_tensor* x,
+                     const nsmi_lanczos_opts* opts, nsmi_solver_info* info) {
+  if (!ctx || !op || !x) return NSMI_ERR_INVALID;
+  API_BEGIN
+  const bool cp = heff_is_complex(op);
+  require(x->dtype == (cp ? NSMI_C128 : NSMI_F64), "expm: x element type does not match the operator");
+  require(x->numel == heff_size(op), "expm: x does not match the operator");
+  require(cp || tau_im == 0.0, "expm: a complex time step needs a complex state");
+  const int K = opts ? opts->krylovdim : 30;
+  const int maxiter = opts ? opts->maxiter : 100;
+  const double tol = opts ? opts->tol : 1e-12;
+  const double eta = opts ? opts->eta : 1.0 / std::sqrt(2.0);
+  require(K >= 1 && K <= 32, "expm: krylovdim must be in [1, 32]");
+  KOps ops{ctx, x->numel, cp};
+  const size_t nd = ops.doubles();
+  double* xd = static_cast<double*>(x->data);
+  const cplx tau(tau_re, tau_im);
+  const double total = std::abs(tau);
+  int numops = 0, kmax = 0, substeps = 0;
+  double last_err = 0.0;
+  if (total > 0.0) {
+    const cplx sgn = tau / total;
+    Scratch basis(ctx, sizeof(double) * nd * (K + 1));
+    double* V = basis.as<double>();
+    double* r = V + nd * K;
+    double done = 0.0, dt_next = total;
+    while (total - done > 1e-14 * total) {
+      if (substeps >= maxiter) fail(NSMI_ERR_SOLVER, "expm: exponentiate did not converge within maxiter sub-steps");
+      ++substeps;
+      const double beta0 = ops.norm(xd);
+      if (beta0 == 0.0) break;
+      double alpha, beta;
+      lanczos_start(ops, op, xd, beta0, V, r, eta, &alpha, &beta);
+      ++numops;
+      std::vector<double> alphas{alpha}, betas;
+      int k = 1;
+      double dt = std::min(total - done, dt_next);
+      dt_next = total;
+      std::vector<cplx> y;
+      for (;;) {
+        std::vector<double> w, S;
+        tridiag_eigh(alphas, betas, w, S);
+        auto evaluate = [&](double step, std::vector<cplx>& yy) {
+          const cplx z = sgn * step;
+          yy.assign(k, 0.0);
+          cplx phik = 0.0;
+          for (int m = 0; m < k; ++m) {
+            const cplx em = std::exp(z * w[m]) * S[0 * k + m];
+            const cplx pm = phi1(z * w[m]) * S[0 * k + m];
+            for (int j = 0; j < k; ++j) yy[j] += S[j * k + m] * em;
+            phik += S[(k - 1) * k + m] * pm;
+          }
+          return beta0 * beta * step * std::abs(phik);
+        };
+        double err = evaluate(dt, y);
+        if (beta <= 1e-14 * beta0 || err <= tol * dt) {
+          last_err = err;
+          break;
+        }
+        if (k == K) {  // shrink by the error model err ~ dt^(K+1) until accepted
+          for (int tries = 0; err > tol * dt && tries < 60; ++tries) {
+            dt *= std::min(0.9, std::max(0.1, 0.9 * std::pow(tol * dt / err, 1.0 / K)));
+            err = evaluate(dt, y);
+          }
+          dt_next = dt;
+          last_err = err;
+          break;
+        }
+        double a, b;
+        betas.push_back(beta);
+        lanczos_expand(ops, op, V, k, r, beta, eta, &a, &b);
+        ++numops;
+        alphas.push_back(a);
+        beta = b;
+        ++k;
+      }
+      kmax = std::max(kmax, k);
+      std::vector<const double*> Vp(k);
+      std::vector<cplx> c(k);
+      for (int j = 0; j < k; ++j) {
+        Vp[j] = V + nd * j;
+        c[j] = beta0 * y[j];
+      }
+      ops.lincomb(xd, Vp, c);  // real tau on a real state keeps every coefficient real
+      done += dt;
+    }
+  }
+  if (info) {
+    info->numops = numops;
+    info->krylov_dim = kmax;
+    info->normres = last_err;
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  API_END(ctx)
+}
+
+}  // extern "C"

@@ -20,6 +20,7 @@ struct nsmi_heff {
   const nsmi_tensor* R = nullptr;
   int precision = NSMI_PREC_FP64;
   int nsite = 2;
+  bool cplx = false;  // ComplexF64 environments and states (real MPO)
   int64_t cl = 0, cr = 0, D0 = 0, D2 = 0, dphys = 0;  // dphys = d1*d2 (2-site) or d
   int P = 0, Q = 0;
   double* Mx = nullptr;  // device P x Q mixing matrix
@@ -321,6 +322,22 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
   const int64_t M2 = op->cl * op->dphys, K2 = op->D2 * op->cr, N2 = op->cr;
   GemmDesc g1 = desc(M1, N1, K1, false, false);
   GemmDesc g2 = desc(M2, N2, K2, false, true);
+  if (op->cplx) {
+    // complex GEMMs; the real MPO mixing acts on interleaved (re, im) pairs as on
+    // a real array with a doubled innermost extent
+    double* T1 = static_cast<double*>(op->T1);
+    double* T3 = static_cast<double*>(op->T3);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, 4.0 * op->flops_gemm1, 16.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      gemm_c128(ctx, g1, dptr(op->L), false, x, false, T1, kRoleHeff1);
+    }
+    mix_f64(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, 2 * op->cr, NSMI_K_HEFF_MIX);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, 4.0 * op->flops_gemm2, 16.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      gemm_c128(ctx, g2, T3, false, dptr(op->R), false, y, kRoleHeff2);
+    }
+    return;
+  }
   if (prec == NSMI_PREC_FP64) {
     double* T1 = static_cast<double*>(op->T1);
     double* T3 = static_cast<double*>(op->T3);
@@ -373,6 +390,14 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
   fail(NSMI_ERR_UNSUPPORTED, "H_eff precision not served by this build");
 }
 
+namespace nsmi {
+void heff_apply_dev(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y) {
+  heff_apply_impl(ctx, op, x, y, op->precision);
+}
+int64_t heff_size(const nsmi_heff* op) { return op->n; }
+bool heff_is_complex(const nsmi_heff* op) { return op->cplx; }
+}  // namespace nsmi
+
 static void heff_release(nsmi_ctx* ctx, nsmi_heff* op) {
   void* bufs[] = {op->Mx, op->T1, op->T3, op->L32, op->R32, op->Lhi, op->Llo, op->Rhi, op->Rlo,
                   op->xThi, op->xTlo, op->T3hi, op->T3lo, op->T1f};
@@ -401,14 +426,22 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   nsmi_heff* op = nullptr;
   API_BEGIN
   *out = nullptr;
-  check_f64(L, "L");
-  check_f64(R, "R");
-  check_f64(W1, "W1");
+  require(L && R, "heff: L and R are required");
+  const bool cplx = L->dtype == NSMI_C128;
+  if (!cplx) {
+    check_f64(L, "L");
+    check_f64(R, "R");
+  } else {
+    require(R->dtype == NSMI_C128, "heff: L and R must have the same element type");
+    if (precision != NSMI_PREC_FP64) fail(NSMI_ERR_UNSUPPORTED, "heff: complex operators are served in FP64 only");
+  }
+  check_f64(W1, "W1 (the MPO is real)");
   require(L->rank == 3 && R->rank == 3 && W1->rank == 4, "heff: L, R must be rank 3 and W rank 4");
   require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32 || precision == NSMI_PREC_BF16X3 ||
               precision == NSMI_PREC_BF16,
           "heff: unknown precision");
   op = new nsmi_heff();
+  op->cplx = cplx;
   op->L = L;
   op->R = R;
   op->precision = precision;
@@ -449,7 +482,7 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   const bool bf = precision == NSMI_PREC_BF16X3 || precision == NSMI_PREC_BF16;
   const bool two = precision == NSMI_PREC_BF16X3;
   if (!bf) {
-    const size_t elt = prec_elt(precision);
+    const size_t elt = cplx ? 16 : prec_elt(precision);
     op->T1 = scratch(ctx, elt * size_t(op->cl * op->D0 * op->dphys * op->cr));
     op->T3 = scratch(ctx, elt * size_t(op->cl * op->D2 * op->dphys * op->cr));
   } else {
@@ -511,19 +544,30 @@ int nsmi_heff_destroy(nsmi_ctx* ctx, nsmi_heff* op) {
 int nsmi_heff_apply(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y) {
   if (!ctx || !op) return NSMI_ERR_INVALID;
   API_BEGIN
-  check_f64(x, "x");
-  check_f64(y, "y");
+  const int dt = op->cplx ? NSMI_C128 : NSMI_F64;
+  require(x && y && x->dtype == dt && y->dtype == dt, "heff_apply: x/y element type does not match the operator");
   require(x->numel == op->n && y->numel == op->n, "heff_apply: x/y size does not match the operator");
   require(x->data != y->data, "heff_apply: x and y must not alias");
-  heff_apply_impl(ctx, op, dptr(x), dptr(y), op->precision);
+  heff_apply_impl(ctx, op, static_cast<const double*>(x->data), static_cast<double*>(y->data), op->precision);
   API_END(ctx)
 }
 
 int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, double* e) {
   if (!ctx || !op || !e) return NSMI_ERR_INVALID;
   API_BEGIN
-  check_f64(x, "x");
+  require(x && x->dtype == (op->cplx ? NSMI_C128 : NSMI_F64), "rayleigh: element type does not match");
   require(x->numel == op->n, "rayleigh: size mismatch");
+  if (op->cplx) {
+    Scratch ys(ctx, 16 * size_t(op->n));
+    heff_apply_impl(ctx, op, static_cast<const double*>(x->data), ys.as<double>(), NSMI_PREC_FP64);
+    vec_zdot_dev(ctx, static_cast<const double*>(x->data), ys.as<double>(), op->n, 0);
+    vec_dot_dev(ctx, static_cast<const double*>(x->data), static_cast<const double*>(x->data), 2 * op->n, 2);
+    double r[3];
+    read_slots(ctx, 3, r);
+    require(r[2] > 0.0, "rayleigh: zero vector");
+    *e = r[0] / r[2];
+    return NSMI_OK;
+  }
   // an FP64 apply needs FP64-sized workspace
   void* T1 = op->T1;
   void* T3 = op->T3;
@@ -558,14 +602,59 @@ int nsmi_env_update(nsmi_ctx* ctx, const nsmi_tensor* E, const nsmi_tensor* A, c
   if (!ctx || !out) return NSMI_ERR_INVALID;
   API_BEGIN
   *out = nullptr;
-  check_f64(E, "E");
-  check_f64(A, "A");
-  check_f64(W, "W");
+  require(E && A, "env_update: E and A are required");
+  const bool cplx = A->dtype == NSMI_C128;
+  if (cplx)
+    require(E->dtype == NSMI_C128, "env_update: E and A must have the same element type");
+  else {
+    check_f64(E, "E");
+    check_f64(A, "A");
+  }
+  check_f64(W, "W (the MPO is real)");
   require(E->rank == 3 && A->rank == 3 && W->rank == 4, "env_update: ranks must be E:3, A:3, W:4");
   require(side == 0 || side == 1, "env_update: side must be 0 (left) or 1 (right)");
-  require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32, "env_update: precision not served");
+  if (precision != NSMI_PREC_FP64) fail(NSMI_ERR_UNSUPPORTED, "env_update: environments are computed in FP64");
   const int64_t ca = A->dims[0], d = A->dims[1], cb = A->dims[2];
   require(W->dims[2] == d && W->dims[3] == d, "env_update: W physical dims do not match A");
+  if (cplx) {
+    const int64_t Dl = W->dims[0], Dr = W->dims[1];
+    const double* Ad = static_cast<const double*>(A->data);
+    const double* Ed = static_cast<const double*>(E->data);
+    nsmi_tensor* res = nullptr;
+    if (side == 0) {
+      require(E->dims[0] == ca && E->dims[2] == ca && E->dims[1] == Dl, "env_update(left): E does not match A/W");
+      const int Q = int(Dl * d), P = int(d * Dr);
+      require(P <= 64 && Q <= 64, "env_update: D*d above 64 is not served");
+      Scratch dmx(ctx, size_t(P) * Q * 8);
+      build_w1(ctx, dptr(W), dmx.as<double>(), int(Dl), int(Dr), int(d), 0);
+      const int64_t dims[3] = {cb, Dr, cb};
+      res = tensor_new(ctx, NSMI_C128, 3, dims);
+      Scratch X(ctx, 16 * size_t(ca * Dl * d * cb)), Y(ctx, 16 * size_t(ca * d * Dr * cb));
+      ProfScope ps(ctx, NSMI_K_ENV, 4.0 * (2.0 * ca * Dl * ca * d * cb + 2.0 * cb * Dr * cb * ca * d),
+                   16.0 * (E->numel + 2 * A->numel + res->numel));
+      gemm_c128(ctx, desc(ca * Dl, d * cb, ca, false, false), Ed, false, Ad, false, X.as<double>(), kRoleEnv);
+      mix_f64(ctx, X.as<double>(), Y.as<double>(), dmx.as<double>(), ca, P, Q, 2 * cb, -1);
+      gemm_c128(ctx, desc(cb, Dr * cb, ca * d, true, false), Ad, true, Y.as<double>(), false,
+                static_cast<double*>(res->data), kRoleEnv);
+    } else {
+      require(E->dims[0] == cb && E->dims[2] == cb && E->dims[1] == Dr, "env_update(right): E does not match A/W");
+      const int Q = int(d * Dr), P = int(Dl * d);
+      require(P <= 64 && Q <= 64, "env_update: D*d above 64 is not served");
+      Scratch dmx(ctx, size_t(P) * Q * 8);
+      build_w1(ctx, dptr(W), dmx.as<double>(), int(Dl), int(Dr), int(d), 1);
+      const int64_t dims[3] = {ca, Dl, ca};
+      res = tensor_new(ctx, NSMI_C128, 3, dims);
+      Scratch X(ctx, 16 * size_t(ca * d * Dr * cb)), Y(ctx, 16 * size_t(ca * Dl * d * cb));
+      ProfScope ps(ctx, NSMI_K_ENV, 4.0 * (2.0 * ca * d * cb * Dr * cb + 2.0 * ca * Dl * ca * d * cb),
+                   16.0 * (E->numel + 2 * A->numel + res->numel));
+      gemm_c128(ctx, desc(ca * d, Dr * cb, cb, false, false), Ad, true, Ed, false, X.as<double>(), kRoleEnv);
+      mix_f64(ctx, X.as<double>(), Y.as<double>(), dmx.as<double>(), ca, P, Q, 2 * cb, -1);
+      gemm_c128(ctx, desc(ca * Dl, ca, d * cb, false, true), Y.as<double>(), false, Ad, false,
+                static_cast<double*>(res->data), kRoleEnv);
+    }
+    *out = res;
+    return NSMI_OK;
+  }
   const int64_t Dl = W->dims[0], Dr = W->dims[1];
   nsmi_tensor* res = nullptr;
   if (side == 0) {
@@ -605,11 +694,21 @@ int nsmi_theta(nsmi_ctx* ctx, const nsmi_tensor* A, const nsmi_tensor* B, nsmi_t
   if (!ctx || !theta) return NSMI_ERR_INVALID;
   API_BEGIN
   *theta = nullptr;
-  check_f64(A, "A");
-  check_f64(B, "B");
+  require(A && B, "theta: A and B are required");
   require(A->rank == 3 && B->rank == 3 && A->dims[2] == B->dims[0], "theta: A[l,s,m] and B[m,s,r] required");
   const int64_t cl = A->dims[0], d1 = A->dims[1], cm = A->dims[2], d2 = B->dims[1], cr = B->dims[2];
   const int64_t dims[4] = {cl, d1, d2, cr};
+  if (A->dtype == NSMI_C128 || B->dtype == NSMI_C128) {
+    require(A->dtype == B->dtype, "theta: A and B must have the same element type");
+    nsmi_tensor* t = tensor_new(ctx, NSMI_C128, 4, dims);
+    ProfScope ps(ctx, NSMI_K_THETA, 8.0 * cl * d1 * cm * d2 * cr, 16.0 * (A->numel + B->numel + t->numel));
+    gemm_c128(ctx, desc(cl * d1, d2 * cr, cm, false, false), static_cast<const double*>(A->data), false,
+              static_cast<const double*>(B->data), false, static_cast<double*>(t->data), kRoleTheta);
+    *theta = t;
+    return NSMI_OK;
+  }
+  check_f64(A, "A");
+  check_f64(B, "B");
   nsmi_tensor* t = tensor_new(ctx, NSMI_F64, 4, dims);
   ProfScope ps(ctx, NSMI_K_THETA, 2.0 * cl * d1 * cm * d2 * cr, 8.0 * (A->numel + B->numel + t->numel));
   gemm_f64(ctx, desc(cl * d1, d2 * cr, cm, false, false), 1.0, dptr(A), dptr(B), 0.0, dptr(t), kRoleTheta);
@@ -617,121 +716,5 @@ int nsmi_theta(nsmi_ctx* ctx, const nsmi_tensor* A, const nsmi_tensor* B, nsmi_t
   API_END(ctx)
 }
 
-// ------------------------------------------------------------------ Lanczos
-// Restates KrylovKit eigsolve(f, x0, 1, :SR; ishermitian=true, krylovdim, maxiter=1,
-// tol, eager=false) with the ModifiedGramSchmidtIR(eta) orthogonaliser
-// (local_solvers/eigsolve.jl:3-29).  Scalars come back to the host once per
-// reduction; the vectors never leave HBM.
-int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_lanczos_opts* opts, double* eigval,
-                     nsmi_solver_info* info) {
-  if (!ctx || !op || !x || !eigval) return NSMI_ERR_INVALID;
-  API_BEGIN
-  check_f64(x, "x");
-  require(x->numel == op->n, "lanczos: x does not match the operator");
-  int K = opts ? opts->krylovdim : 3;
-  const int maxiter = opts ? opts->maxiter : 1;
-  const double tol = opts ? opts->tol : 1e-14;
-  const double eta = opts ? opts->eta : 1.0 / std::sqrt(2.0);
-  require(K >= 1 && K <= 32, "lanczos: krylovdim must be in [1, 32]");
-  if (maxiter != 1) fail(NSMI_ERR_UNSUPPORTED, "lanczos: only maxiter = 1 is served");
-  const int64_t n = op->n;
-  const double epsv = 2.220446049250313e-16;
-  Scratch basis(ctx, sizeof(double) * size_t(n) * (K + 1));
-  double* V = basis.as<double>();
-  double* r = V + size_t(n) * K;  // residual / work vector
-  double* xd = dptr(x);
-
-  double s[2];
-  vec_dot_dev(ctx, xd, xd, n, 0);
-  read_slots(ctx, 1, s);
-  const double beta0 = std::sqrt(s[0]);
-  require(beta0 > 0.0, "lanczos: initial vector should not have norm zero");
-  // Ax0 -> r
-  heff_apply_impl(ctx, op, xd, r, op->precision);
-  int numops = 1;
-  vec_dot_dev(ctx, xd, r, n, 0);
-  read_slots(ctx, 1, s);
-  double alpha = s[0] / (beta0 * beta0);
-  vec_scaled_copy(ctx, V, xd, n, 1.0 / beta0);   // v1 = x0 / beta0
-  vec_scale(ctx, r, n, 1.0 / beta0);             // r = Ax0 / beta0
-  vec_dot_dev(ctx, r, r, n, 0);
-  read_slots(ctx, 1, s);
-  double beta_old = std::sqrt(s[0]);
-  vec_axpy2_norm2_dev(ctx, r, V, -alpha, nullptr, 0.0, n, 0);
-  read_slots(ctx, 1, s);
-  double beta = std::sqrt(s[0]);
-  while (epsv < beta && beta < eta * beta_old) {
-    beta_old = beta;
-    vec_dot_dev(ctx, V, r, n, 0);
-    read_slots(ctx, 1, s);
-    alpha += s[0];
-    vec_axpy2_norm2_dev(ctx, r, V, -s[0], nullptr, 0.0, n, 0);
-    read_slots(ctx, 1, s);
-    beta = std::sqrt(s[0]);
-  }
-  std::vector<double> alphas{alpha}, betas;
-  int k = 1;
-  while (!(k == K || beta <= tol)) {
-    double* vk = V + size_t(n) * k;
-    double* vprev = V + size_t(n) * (k - 1);
-    vec_scaled_copy(ctx, vk, r, n, 1.0 / beta);
-    betas.push_back(beta);
-    heff_apply_impl(ctx, op, vk, r, op->precision);  // w
-    ++numops;
-    // w -= beta * v_{k-1}; a = <v_k, w>
-    vec_axpy(ctx, r, vprev, n, -beta);
-    vec_dot_dev(ctx, vk, r, n, 0);
-    read_slots(ctx, 1, s);
-    double a = s[0];
-    vec_axpy2_norm2_dev(ctx, r, vk, -a, nullptr, 0.0, n, 0);
-    read_slots(ctx, 1, s);
-    double b = std::sqrt(s[0]);
-    double nold = std::sqrt(b * b + a * a + beta * beta);
-    while (epsv < b && b < eta * nold) {
-      nold = b;
-      double c = 0.0;
-      for (int j = 0; j <= k; ++j) {  // modified Gram-Schmidt against the whole basis
-        const double* q = V + size_t(n) * j;
-        vec_dot_dev(ctx, q, r, n, 0);
-        read_slots(ctx, 1, s);
-        c = s[0];
-        vec_axpy(ctx, r, q, n, -c);
-      }
-      a += c;
-      vec_dot_dev(ctx, r, r, n, 0);
-      read_slots(ctx, 1, s);
-      b = std::sqrt(s[0]);
-    }
-    alphas.push_back(a);
-    beta = b;
-    ++k;
-  }
-  // projected tridiagonal problem
-  std::vector<double> T(size_t(k) * k, 0.0), w(k), Y(size_t(k) * k);
-  for (int i = 0; i < k; ++i) T[i * k + i] = alphas[i];
-  for (int i = 0; i + 1 < k; ++i) T[i * k + i + 1] = T[(i + 1) * k + i] = betas[i];
-  small_sym_eigh(k, T.data(), w.data(), Y.data());
-  std::vector<double> y(k);
-  double ymax = 0.0;
-  for (int i = 0; i < k; ++i) {
-    y[i] = Y[i * k + 0];
-    ymax = std::max(ymax, std::fabs(y[i]));
-  }
-  int i0 = 0;
-  while (i0 < k - 1 && !(std::fabs(y[i0]) > 1e-14 * ymax)) ++i0;
-  if (y[i0] < 0)
-    for (auto& v : y) v = -v;
-  std::vector<const double*> Vp(k);
-  for (int j = 0; j < k; ++j) Vp[j] = V + size_t(n) * j;
-  vec_lincomb(ctx, xd, Vp.data(), y.data(), k, n);
-  *eigval = w[0];
-  if (info) {
-    info->numops = numops;
-    info->krylov_dim = k;
-    info->normres = std::fabs(beta * y[k - 1]);
-  }
-  NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  API_END(ctx)
-}
 
 }  // extern "C"

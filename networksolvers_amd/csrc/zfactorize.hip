@@ -1,0 +1,339 @@
+// Complex (C128) truncated factorisation and QR for the TDVP path.
+//
+// Gram route for both "svd" and "eigen": rho = M M^H (isometry on the rows) or
+// rho' = M^T conj(M) (isometry on the columns), rocSOLVER zheevd, NDTensors
+// truncate! on the host, then the kept eigenvectors with the phase gauge "largest-
+// magnitude entry real and positive".  zheevd sees the row-major buffer as the
+// column-major conjugate of rho, so row j of its output is z_j = conj(u_j); with
+// q_j the phase of z_j at its first largest-magnitude index, Zq[j] = z_j conj(q_j):
+//   rows:    U = Zq^H (m x k),        C = Zq . M         (k x nc)
+//   columns: U^T = conj(Zq) (k x nc), C^T = M . Zq^T     (m x k)
+// QR: zgelqf/zunglq (rows) or zgeqrf/zungqr (columns) with the phase of diag(R)
+// moved into Q so that diag(R) is real and non-negative.
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "internal.h"
+
+namespace nsmi {
+namespace {
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+inline unsigned zblocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
+
+// out[j][i] = X[idx[j]][i] * conj(q_j), q_j = phase of row idx[j] at its first
+// largest-magnitude entry.  One block per kept row.
+__global__ void k_zgather_phase_rows(const double* __restrict__ X, int64_t ld, const int* __restrict__ idx,
+                                     int64_t len, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const d2* row = reinterpret_cast<const d2*>(X) + int64_t(idx ? idx[j] : j) * ld;
+  double best = -1.0;
+  int64_t bi = 0;
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const d2 z = row[i];
+    const double a = z.x * z.x + z.y * z.y;
+    if (a > best) {
+      best = a;
+      bi = i;
+    }
+  }
+  __shared__ double sb[256];
+  __shared__ int64_t si[256];
+  sb[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      const double b2 = sb[threadIdx.x + o];
+      const int64_t i2 = si[threadIdx.x + o];
+      if (b2 > sb[threadIdx.x] || (b2 == sb[threadIdx.x] && i2 < si[threadIdx.x])) {
+        sb[threadIdx.x] = b2;
+        si[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  const d2 piv = row[si[0]];
+  const double mag = sqrt(piv.x * piv.x + piv.y * piv.y);
+  const double qr = mag > 0 ? piv.x / mag : 1.0, qi = mag > 0 ? piv.y / mag : 0.0;  // q_j
+  d2* o = reinterpret_cast<d2*>(out) + int64_t(j) * len;
+  for (int64_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const d2 z = row[i];
+    o[i] = d2{z.x * qr + z.y * qi, z.y * qr - z.x * qi};  // z * conj(q)
+  }
+}
+
+// out (len x k) = in (k x len)^H
+__global__ void k_zconj_transpose(const double* __restrict__ in, int k, int64_t len, double* __restrict__ out) {
+  __shared__ d2 tile[32][33];
+  const int64_t i0 = int64_t(blockIdx.x) * 32, j0 = int64_t(blockIdx.y) * 32;
+  const d2* src = reinterpret_cast<const d2*>(in);
+  d2* dst = reinterpret_cast<d2*>(out);
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int64_t j = j0 + r, i = i0 + threadIdx.x;
+    if (j < k && i < len) tile[r][threadIdx.x] = src[j * len + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int64_t i = i0 + r, j = j0 + threadIdx.x;
+    if (i < len && j < k) {
+      const d2 z = tile[threadIdx.x][r];
+      dst[i * k + j] = d2{z.x, -z.y};
+    }
+  }
+}
+
+__global__ void k_zconj(const double* __restrict__ in, int64_t n, double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  out[2 * t] = in[2 * t];
+  out[2 * t + 1] = -in[2 * t + 1];
+}
+
+// phase p_i = R[i][i] / |R[i][i]| of the diagonal of a row-major complex matrix (row stride ld)
+__global__ void k_zdiag_phase(const double* __restrict__ X, int64_t ld, int64_t r, double* __restrict__ ph) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= r) return;
+  const d2 z = reinterpret_cast<const d2*>(X)[i * ld + i];
+  const double mag = sqrt(z.x * z.x + z.y * z.y);
+  ph[2 * i] = mag > 0 ? z.x / mag : 1.0;
+  ph[2 * i + 1] = mag > 0 ? z.y / mag : 0.0;
+}
+
+// R (r x n) = conj(p_i) * X[i][j] for j >= i  (rows of the gelqf output)
+__global__ void k_zextract_tri(const double* __restrict__ X, int64_t ldx, int64_t r, int64_t n,
+                               const double* __restrict__ ph, double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= r * n) return;
+  const int64_t i = t / n, j = t % n;
+  d2 v = {0.0, 0.0};
+  if (j >= i) {
+    const d2 z = reinterpret_cast<const d2*>(X)[i * ldx + j];
+    const double pr = ph[2 * i], pi = ph[2 * i + 1];
+    v = d2{z.x * pr + z.y * pi, z.y * pr - z.x * pi};
+  }
+  reinterpret_cast<d2*>(out)[t] = v;
+}
+
+// R^T (m x r): (j, i) = conj(p_i) * X[j][i] for i <= j  (geqrf output read row-major)
+__global__ void k_zextract_tri_t(const double* __restrict__ X, int64_t ldx, int64_t r, int64_t m,
+                                 const double* __restrict__ ph, double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= r * m) return;
+  const int64_t j = t / r, i = t % r;
+  d2 v = {0.0, 0.0};
+  if (i <= j) {
+    const d2 z = reinterpret_cast<const d2*>(X)[j * ldx + i];
+    const double pr = ph[2 * i], pi = ph[2 * i + 1];
+    v = d2{z.x * pr + z.y * pi, z.y * pr - z.x * pi};
+  }
+  reinterpret_cast<d2*>(out)[t] = v;
+}
+
+// out (len x k) [i][j] = p_j * X[i][j] (row stride ld)  -- Q columns with the diagonal phases
+__global__ void k_zgather_cols_phase(const double* __restrict__ X, int64_t ld, int k, int64_t len,
+                                     const double* __restrict__ ph, double* __restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(k) * len) return;
+  const int64_t i = t / k;
+  const int j = int(t % k);
+  const d2 z = reinterpret_cast<const d2*>(X)[i * ld + j];
+  const double pr = ph[2 * j], pi = ph[2 * j + 1];
+  reinterpret_cast<d2*>(out)[t] = d2{z.x * pr - z.y * pi, z.x * pi + z.y * pr};
+}
+
+// rows of X (rows x cols) times p_row
+__global__ void k_zscale_rows_phase(double* __restrict__ X, int64_t rows, int64_t cols,
+                                    const double* __restrict__ ph) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= rows * cols) return;
+  const int64_t i = t / cols;
+  const d2 z = reinterpret_cast<const d2*>(X)[t];
+  const double pr = ph[2 * i], pi = ph[2 * i + 1];
+  reinterpret_cast<d2*>(X)[t] = d2{z.x * pr - z.y * pi, z.x * pi + z.y * pr};
+}
+
+GemmDesc zdesc(int64_t M, int64_t N, int64_t K, bool ta, bool tb) {
+  GemmDesc g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.transA = ta;
+  g.transB = tb;
+  g.lda = ta ? M : K;
+  g.ldb = tb ? K : N;
+  g.ldc = N;
+  g.batch = 1;
+  return g;
+}
+
+void zcheck(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success)
+    fail(NSMI_ERR_SOLVER, std::string(what) + " failed with rocblas_status " + std::to_string(int(st)));
+}
+
+void zcheck_info(nsmi_ctx* ctx, const char* what) {
+  int info = 0;
+  NSMI_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  if (info != 0) fail(NSMI_ERR_SOLVER, std::string(what) + " returned info = " + std::to_string(info));
+}
+
+}  // namespace
+
+void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing, const nsmi_trunc* trunc,
+                int route, int64_t maxdim, int64_t mindim, nsmi_tensor** Uout, nsmi_tensor** Cout, int64_t* kept,
+                double* truncerr, double* total,
+                int64_t (*truncate)(std::vector<double>&, int64_t, int64_t, double, double*, double*)) {
+  int64_t m = 1, nc = 1;
+  for (int i = 0; i < n_row_dims; ++i) m *= theta->dims[i];
+  for (int i = n_row_dims; i < theta->rank; ++i) nc *= theta->dims[i];
+  const int64_t iso = rows_trailing ? nc : m, oth = rows_trailing ? m : nc;
+  const double* M = static_cast<const double*>(theta->data);
+  Scratch rho(ctx, 16 * size_t(iso) * iso);
+  Scratch w(ctx, 8 * size_t(iso));
+  Scratch e(ctx, 8 * size_t(iso));
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_GEMM, 8.0 * double(iso) * iso * oth, 16.0 * (double(m) * nc + double(iso) * iso));
+    if (!rows_trailing)
+      gemm_c128(ctx, zdesc(m, m, nc, false, true), M, false, M, true, rho.as<double>(), kRoleFact);
+    else
+      gemm_c128(ctx, zdesc(nc, nc, m, true, false), M, false, M, true, rho.as<double>(), kRoleFact);
+  }
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
+    zcheck(rocsolver_zheevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
+                            reinterpret_cast<rocblas_double_complex*>(rho.p), rocblas_int(iso), w.as<double>(),
+                            e.as<double>(), ctx->d_info),
+           "rocsolver_zheevd");
+  }
+  zcheck_info(ctx, "rocsolver_zheevd");
+  std::vector<double> wh(iso);
+  NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, 8 * iso, hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  for (double v : wh)
+    if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
+  std::vector<int> order(iso);
+  std::iota(order.begin(), order.end(), 0);
+  if (route == NSMI_ROUTE_EIGEN) {
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
+  } else {
+    for (auto& v : wh) v = std::max(v, 0.0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wh[a] > wh[b]; });
+  }
+  std::vector<double> P(iso);
+  for (int64_t i = 0; i < iso; ++i) P[i] = wh[order[i]];
+  const int64_t k = truncate(P, mindim, maxdim, trunc->cutoff, truncerr, total);
+  Scratch d_idx(ctx, sizeof(int) * std::max<int64_t>(k, 1));
+  NSMI_HIP(hipMemcpyAsync(d_idx.p, order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+  Scratch Zq(ctx, 16 * size_t(k) * iso);
+  hipLaunchKernelGGL(k_zgather_phase_rows, dim3(unsigned(k)), dim3(256), 0, ctx->stream, rho.as<double>(), iso,
+                     d_idx.as<int>(), iso, Zq.as<double>());
+  NSMI_LAUNCH_CHECK();
+  int64_t ud[8], cd[8];
+  int ur = 0, cr = 0;
+  if (!rows_trailing) {
+    for (int i = 0; i < n_row_dims; ++i) ud[ur++] = theta->dims[i];
+    ud[ur++] = k;
+    cd[cr++] = k;
+    for (int i = n_row_dims; i < theta->rank; ++i) cd[cr++] = theta->dims[i];
+  } else {
+    ud[ur++] = k;
+    for (int i = n_row_dims; i < theta->rank; ++i) ud[ur++] = theta->dims[i];
+    for (int i = 0; i < n_row_dims; ++i) cd[cr++] = theta->dims[i];
+    cd[cr++] = k;
+  }
+  nsmi_tensor* U = tensor_new(ctx, NSMI_C128, ur, ud);
+  nsmi_tensor* C = tensor_new(ctx, NSMI_C128, cr, cd);
+  ProfScope ps(ctx, NSMI_K_FACT_GEMM, 8.0 * double(k) * m * nc, 16.0 * (double(m) * nc + double(k) * (m + nc)));
+  if (!rows_trailing) {
+    hipLaunchKernelGGL(k_zconj_transpose, dim3(zblocks(m, 32), zblocks(k, 32)), dim3(32, 8), 0, ctx->stream,
+                       Zq.as<double>(), int(k), m, static_cast<double*>(U->data));
+    NSMI_LAUNCH_CHECK();
+    gemm_c128(ctx, zdesc(k, nc, m, false, false), Zq.as<double>(), false, M, false, static_cast<double*>(C->data),
+              kRoleFact);
+  } else {
+    hipLaunchKernelGGL(k_zconj, dim3(zblocks(k * nc)), dim3(256), 0, ctx->stream, Zq.as<double>(), k * nc,
+                       static_cast<double*>(U->data));
+    NSMI_LAUNCH_CHECK();
+    gemm_c128(ctx, zdesc(m, k, nc, false, true), M, false, Zq.as<double>(), false, static_cast<double*>(C->data),
+              kRoleFact);
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  *Uout = U;
+  *Cout = C;
+  *kept = k;
+}
+
+void zqr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Qout,
+         nsmi_tensor** Rout) {
+  int64_t m = 1, nc = 1;
+  for (int i = 0; i < n_row_dims; ++i) m *= A->dims[i];
+  for (int i = n_row_dims; i < A->rank; ++i) nc *= A->dims[i];
+  const int64_t xr = rows_trailing ? nc : m, xc = rows_trailing ? m : nc;
+  const int64_t r = std::min(xr, xc);
+  Scratch a(ctx, 16 * size_t(m) * nc);
+  Scratch tau(ctx, 16 * size_t(std::max<int64_t>(r, 1)));
+  Scratch ph(ctx, 16 * size_t(std::max<int64_t>(r, 1)));
+  NSMI_HIP(hipMemcpyAsync(a.p, A->data, 16 * size_t(m) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+  auto* az = reinterpret_cast<rocblas_double_complex*>(a.p);
+  auto* tz = reinterpret_cast<rocblas_double_complex*>(tau.p);
+  int64_t qd[8], rd[8];
+  int qrk = 0, rrk = 0;
+  if (!rows_trailing) {
+    for (int i = 0; i < n_row_dims; ++i) qd[qrk++] = A->dims[i];
+    qd[qrk++] = r;
+    rd[rrk++] = r;
+    for (int i = n_row_dims; i < A->rank; ++i) rd[rrk++] = A->dims[i];
+  } else {
+    qd[qrk++] = r;
+    for (int i = n_row_dims; i < A->rank; ++i) qd[qrk++] = A->dims[i];
+    for (int i = 0; i < n_row_dims; ++i) rd[rrk++] = A->dims[i];
+    rd[rrk++] = r;
+  }
+  nsmi_tensor* Q = tensor_new(ctx, NSMI_C128, qrk, qd);
+  nsmi_tensor* R = tensor_new(ctx, NSMI_C128, rrk, rd);
+  try {
+    if (!rows_trailing) {
+      zcheck(rocsolver_zgelqf(ctx->blas, rocblas_int(nc), rocblas_int(m), az, rocblas_int(nc), tz), "rocsolver_zgelqf");
+      hipLaunchKernelGGL(k_zdiag_phase, dim3(zblocks(r)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r,
+                         ph.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_zextract_tri, dim3(zblocks(r * nc)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r, nc,
+                         ph.as<double>(), static_cast<double*>(R->data));
+      NSMI_LAUNCH_CHECK();
+      zcheck(rocsolver_zunglq(ctx->blas, rocblas_int(r), rocblas_int(m), rocblas_int(r), az, rocblas_int(nc), tz),
+             "rocsolver_zunglq");
+      hipLaunchKernelGGL(k_zgather_cols_phase, dim3(zblocks(r * m)), dim3(256), 0, ctx->stream, a.as<double>(), nc,
+                         int(r), m, ph.as<double>(), static_cast<double*>(Q->data));
+      NSMI_LAUNCH_CHECK();
+    } else {
+      zcheck(rocsolver_zgeqrf(ctx->blas, rocblas_int(nc), rocblas_int(m), az, rocblas_int(nc), tz), "rocsolver_zgeqrf");
+      hipLaunchKernelGGL(k_zdiag_phase, dim3(zblocks(r)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r,
+                         ph.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_zextract_tri_t, dim3(zblocks(r * m)), dim3(256), 0, ctx->stream, a.as<double>(), nc, r, m,
+                         ph.as<double>(), static_cast<double*>(R->data));
+      NSMI_LAUNCH_CHECK();
+      zcheck(rocsolver_zungqr(ctx->blas, rocblas_int(nc), rocblas_int(r), rocblas_int(r), az, rocblas_int(nc), tz),
+             "rocsolver_zungqr");
+      NSMI_HIP(hipMemcpyAsync(Q->data, a.p, 16 * size_t(r) * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_zscale_rows_phase, dim3(zblocks(r * nc)), dim3(256), 0, ctx->stream,
+                         static_cast<double*>(Q->data), r, nc, ph.as<double>());
+      NSMI_LAUNCH_CHECK();
+    }
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    tensor_delete(ctx, Q);
+    tensor_delete(ctx, R);
+    throw;
+  }
+  *Qout = Q;
+  *Rout = R;
+}
+
+}  // namespace nsmi

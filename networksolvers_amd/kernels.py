@@ -94,6 +94,23 @@ def lanczos_eig(H: HeffMap, x: DeviceTensor, *, krylovdim=3, maxiter=1, tol=1e-1
 
 
 @dataclass
+class ExpmInfo:
+    numops: int
+    krylov_dim: int
+    error_estimate: float
+
+
+def krylov_expm(H: HeffMap, x: DeviceTensor, tau, *, krylovdim=30, maxiter=100, tol=1e-12,
+                eta=1 / math.sqrt(2)) -> ExpmInfo:
+    """x <- exp(tau H) x (KrylovKit exponentiate semantics, error per unit time <= tol)."""
+    tau = complex(tau)
+    opts = _lib.LanczosOpts(int(krylovdim), int(maxiter), float(tol), float(eta))
+    info = _lib.SolverInfo()
+    H.ctx.check(H.ctx.lib.nsmi_krylov_expm(H.ctx.h, H.h, tau.real, tau.imag, x.h, C.byref(opts), C.byref(info)))
+    return ExpmInfo(info.numops, info.krylov_dim, info.normres)
+
+
+@dataclass
 class TruncInfo:
     kept: int
     full: int
@@ -124,4 +141,5 @@ def qr(A: DeviceTensor, n_row_dims: int, rows_trailing: bool):
     return DeviceTensor(ctx, Q), DeviceTensor(ctx, R)
 
 
-__all__ = ["Context", "DeviceTensor", "theta", "env_update", "HeffMap", "lanczos_eig", "factorize", "qr"]
+__all__ = ["Context", "DeviceTensor", "theta", "env_update", "HeffMap", "lanczos_eig", "krylov_expm", "factorize",
+           "qr"]

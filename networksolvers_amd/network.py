@@ -33,8 +33,15 @@ class TTN:
         self.stamp = [0] * len(self.tensors)
 
     @classmethod
-    def from_arrays(cls, ctx, arrays, graph=None):
-        return cls(ctx, [np.ascontiguousarray(a, dtype=np.float64) for a in arrays], graph)
+    def from_arrays(cls, ctx, arrays, graph=None, dtype=None):
+        """Upload site tensors; float64 unless any is complex (or ``dtype`` says so)."""
+        if dtype is None:
+            dtype = np.complex128 if any(np.iscomplexobj(a) for a in arrays) else np.float64
+        return cls(ctx, [np.ascontiguousarray(a, dtype=dtype) for a in arrays], graph)
+
+    @property
+    def dtype(self):
+        return self.tensors[0].dtype
 
     def __len__(self):
         return len(self.tensors)
@@ -157,17 +164,18 @@ class ProjTTN:
             out.append((hi + 1, hi))
         return out
 
-    def _boundary(self, mpo_dim):
-        if mpo_dim not in self._one:
-            self._one[mpo_dim] = self.ctx.upload(np.ones((1, mpo_dim, 1)))
-        return self._one[mpo_dim]
+    def _boundary(self, mpo_dim, dtype):
+        key = (mpo_dim, np.dtype(dtype).str)
+        if key not in self._one:
+            self._one[key] = self.ctx.upload(np.ones((1, mpo_dim, 1), dtype=dtype))
+        return self._one[key]
 
     def environment(self, psi: TTN, e):
         """Environment of edge e = (u, v); the trivial one beyond the chain ends."""
         u, v = e
         N = len(self.H)
         if u < 0 or u >= N:
-            return self._boundary(1)
+            return self._boundary(1, psi.dtype)
         ent = self.envs.get(e)
         covered = range(0, u + 1) if v == u + 1 else range(u, N)
         if ent is not None:
@@ -200,24 +208,29 @@ class ProjTTN:
 
 
 def expect_mpo(psi: TTN, H: TTNO, precision="fp64") -> float:
-    """<psi|H|psi> / <psi|psi> by a full left-to-right environment pass (FP64 by default)."""
-    E = H.ctx.upload(np.ones((1, 1, 1)))
+    """<psi|H|psi> / <psi|psi> by a full left-to-right environment pass in FP64."""
+    ctx = H.ctx
+    E = ctx.upload(np.ones((1, 1, 1), dtype=psi.dtype))
     for i in range(len(psi)):
         E = kernels.env_update(E, psi[i], H[i], 0, precision)
-    val = float(E.numpy().reshape(-1)[0])
-    nrm = inner(psi, psi)
-    return val / nrm
+    val = E.numpy().reshape(-1)[0]
+    return float(np.real(val)) / norm2(psi)
 
 
-def inner(a: TTN, b: TTN) -> float:
-    """<a|b> via environment updates with an identity MPO."""
-    ctx = a.ctx
-    E = ctx.upload(np.ones((1, 1, 1)))
-    for i in range(len(a)):
-        d = a[i].shape[1]
-        Wid = ctx.upload(np.eye(d).reshape(1, 1, d, d))
-        if a[i] is b[i]:
-            E = kernels.env_update(E, a[i], Wid, 0)
-        else:
-            raise NotImplementedError("inner of two different states is not served on the device")
-    return float(E.numpy().reshape(-1)[0])
+def norm2(psi: TTN) -> float:
+    """<psi|psi> by a transfer-operator pass (identity MPO)."""
+    ctx = psi.ctx
+    E = ctx.upload(np.ones((1, 1, 1), dtype=psi.dtype))
+    for i in range(len(psi)):
+        d = psi[i].shape[1]
+        E = kernels.env_update(E, psi[i], ctx.upload(np.eye(d).reshape(1, 1, d, d)), 0)
+    return float(np.real(E.numpy().reshape(-1)[0]))
+
+
+def inner(a: TTN, b: TTN):
+    """<a|b> (host contraction of downloaded tensors; for checks and observables)."""
+    E = np.ones((1, 1), dtype=np.complex128)
+    for x, y in zip(a.to_arrays(), b.to_arrays()):
+        E = np.tensordot(x.conj(), np.tensordot(E, y, axes=([1], [0])), axes=([0, 1], [0, 1]))
+    v = E[0, 0]
+    return v if (np.iscomplexobj(a.tensors[0].numpy()) or np.iscomplexobj(b.tensors[0].numpy())) else v.real

@@ -112,9 +112,10 @@ def region_iterator_action(problem, region_iter, *, extracter_kwargs=None, updat
     extracter_kwargs = extracter_kwargs or {}
     updater_kwargs = updater_kwargs or {}
     inserter_kwargs = inserter_kwargs or {}
-    problem, local_state = extracter(problem, region_iter, **extracter_kwargs, sweep=sweep, **kws)
-    problem, local_state = updater(problem, local_state, region_iter, **updater_kwargs, **kws)
-    problem = inserter(problem, local_state, region_iter, sweep=sweep, **inserter_kwargs, **kws)
+    # later keyword packs win, as Julia's splatted keywords merge
+    problem, local_state = extracter(problem, region_iter, **{**extracter_kwargs, "sweep": sweep, **kws})
+    problem, local_state = updater(problem, local_state, region_iter, **{**updater_kwargs, **kws})
+    problem = inserter(problem, local_state, region_iter, **{"sweep": sweep, **inserter_kwargs, **kws})
     return problem
 
 

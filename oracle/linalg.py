@@ -264,9 +264,9 @@ def lanczos_eigsolve(apply, x0, *, krylovdim=3, maxiter=1, tol=1e-14,
     epsv = np.finfo(np.float64).eps
     while epsv < beta < eta * beta_old:
         beta_old = beta
-        da = _inner(v, r).real
-        alpha += da
-        r = r - da * v
+        da = _inner(v, r)
+        alpha += da.real
+        r = r - da * v   # the full (complex) projection is removed; alpha keeps its real part
         beta = float(np.linalg.norm(r))
     V = [v]
     alphas = [alpha]
@@ -283,8 +283,9 @@ def lanczos_eigsolve(apply, x0, *, krylovdim=3, maxiter=1, tol=1e-14,
         w = apply(v_new)
         numops += 1
         w = w - beta * vprev
-        a = _inner(v_new, w).real
-        w = w - a * v_new
+        ca = _inner(v_new, w)
+        w = w - ca * v_new
+        a = ca.real
         ab2 = a * a + beta * beta
         b = float(np.linalg.norm(w))
         nold = math.sqrt(b * b + ab2)

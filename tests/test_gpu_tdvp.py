@@ -1,0 +1,321 @@
+"""GPU parity of the complex (C128) region kernels and of TDVP against the oracle.
+
+Tolerances: complex FP64 contractions agree with the oracle to 1e-12 relative;
+factorize / QR after the shared phase gauge to 1e-10; the Krylov exponential to
+1e-10 relative (the same algorithm, tol 1e-12 per unit time); TDVP states agree
+with the oracle's TDVP to fidelity 1 - 1e-9, and at full bond rank with
+expm(-i H t) psi0 to 1e-9 (the integrator is exact there).
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import chain, linalg, models, tdvp as otdvp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import networksolvers_amd as ns
+    c = ns.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _ed(model, key):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")) as f:
+        return json.load(f)[model][key]
+
+
+def crand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+# ------------------------------------------------------------ region kernels
+@pytest.mark.parametrize("shape", [(1, 2, 3, 2, 1), (5, 2, 7, 2, 3), (37, 3, 64, 3, 29), (130, 2, 129, 2, 131)])
+def test_theta_c128(ctx, shape):
+    import networksolvers_amd as ns
+    cl, d1, cm, d2, cr = shape
+    A, B = crand((cl, d1, cm), 1), crand((cm, d2, cr), 2)
+    t = ns.theta(ctx.upload(A), ctx.upload(B)).numpy()
+    assert t.dtype == np.complex128
+    assert rel(t, chain.theta2(A, B)) < 1e-13
+
+
+@pytest.mark.parametrize("side", [0, 1])
+@pytest.mark.parametrize("dims", [(1, 2, 1, 1, 5), (3, 2, 4, 5, 5), (33, 2, 17, 5, 5), (70, 3, 90, 5, 5),
+                                  (130, 2, 200, 5, 5)])
+def test_env_update_c128(ctx, side, dims):
+    import networksolvers_amd as ns
+    ca, d, cb, Dl, Dr = dims
+    A = crand((ca, d, cb), 3)
+    W = np.random.default_rng(4).standard_normal((Dl, Dr, d, d))
+    if side == 0:
+        E = crand((ca, Dl, ca), 5)
+        want = chain.env_left(E, A, W)
+    else:
+        E = crand((cb, Dr, cb), 5)
+        want = chain.env_right(E, A, W)
+    got = ns.env_update(ctx.upload(E), ctx.upload(A), ctx.upload(W), side).numpy()
+    assert got.shape == want.shape
+    assert rel(got, want) < 1e-12
+
+
+@pytest.mark.parametrize("cl,cr,d,D", [(1, 1, 2, 1), (4, 3, 2, 5), (20, 33, 2, 5), (64, 64, 2, 5),
+                                       (100, 70, 3, 5), (129, 130, 2, 3)])
+def test_heff2_c128(ctx, cl, cr, d, D):
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(7)
+    L, R = crand((cl, D, cl), 8), crand((cr, D, cr), 9)
+    W1, W2 = rng.standard_normal((D, D, d, d)), rng.standard_normal((D, D, d, d))
+    th = crand((cl, d, d, cr), 10)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W1), ctx.upload(W2), ctx.upload(R))
+    assert rel(H(ctx.upload(th)).numpy(), chain.heff2_apply(L, W1, W2, R, th)) < 1e-12
+
+
+@pytest.mark.parametrize("cl,cr,d,D", [(3, 5, 2, 5), (50, 40, 2, 5), (64, 64, 1, 5)])
+def test_heff1_c128(ctx, cl, cr, d, D):
+    import networksolvers_amd as ns
+    L, R = crand((cl, D, cl), 11), crand((cr, D, cr), 12)
+    W = np.random.default_rng(13).standard_normal((D, D, d, d))
+    x = crand((cl, d, cr), 14)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W), None, ctx.upload(R))
+    assert rel(H(ctx.upload(x)).numpy(), chain.heff1_apply(L, W, R, x)) < 1e-12
+
+
+def test_complex_precision_forms_rejected(ctx):
+    import networksolvers_amd as ns
+    L, R = crand((4, 5, 4), 1), crand((4, 5, 4), 2)
+    W = np.zeros((5, 5, 2, 2))
+    with pytest.raises(Exception, match="FP64"):
+        ns.HeffMap(ctx.upload(L), ctx.upload(W), ctx.upload(W), ctx.upload(R), "fp32")
+
+
+def _hermitian_region(N, chi, seed, site):
+    """Environments of a random complex MPS with the real XXZ MPO: a Hermitian H_eff."""
+    W = models.xxz_mpo(N, Jz=0.7, hz=0.3)
+    mps = chain.random_mps(N, 2, chi, seed=seed, dtype=np.complex128)
+    L = np.ones((1, 1, 1), dtype=complex)
+    for i in range(site):
+        L = chain.env_left(L, mps[i], W[i])
+    R = np.ones((1, 1, 1), dtype=complex)
+    for i in range(N - 1, site + 1, -1):
+        R = chain.env_right(R, mps[i], W[i])
+    th = chain.theta2(mps[site], mps[site + 1])
+    return W, L, R, th
+
+
+def test_rayleigh_and_lanczos_c128(ctx):
+    import networksolvers_amd as ns
+    W, L, R, th = _hermitian_region(10, 16, 21, 4)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[4]), ctx.upload(W[5]), ctx.upload(R))
+    x = ctx.upload(th)
+    Hx = chain.heff2_apply(L, W[4], W[5], R, th)
+    want_r = (np.vdot(th, Hx) / np.vdot(th, th)).real
+    assert abs(H.rayleigh(x) - want_r) < 1e-12 * max(1.0, abs(want_r))
+    E0, v0, _ = linalg.lanczos_eigsolve(lambda v: chain.heff2_apply(L, W[4], W[5], R, v.reshape(th.shape)).reshape(-1),
+                                        th.reshape(-1), krylovdim=3)
+    res = ns.lanczos_eig(H, x, krylovdim=3)
+    assert abs(res.eigval - E0) < 1e-11 * max(1.0, abs(E0))
+    got = x.numpy().reshape(-1)
+    assert rel(got, v0) < 1e-10
+
+
+@pytest.mark.parametrize("tau", [-0.1j, -1.5j, 0.2, -0.3 - 0.4j])
+def test_krylov_expm_c128(ctx, tau):
+    import networksolvers_amd as ns
+    W, L, R, th = _hermitian_region(8, 8, 31, 3)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[3]), ctx.upload(W[4]), ctx.upload(R))
+    Hd = chain.heff2_dense_matrix(L, W[3], W[4], R)
+    want = sl.expm(tau * Hd) @ th.reshape(-1)
+    ref, rinfo = otdvp.krylov_expm(lambda v: Hd @ v, tau, th.reshape(-1))
+    x = ctx.upload(th)
+    info = ns.krylov_expm(H, x, tau)
+    got = x.numpy().reshape(-1)
+    assert rel(got, want) < 1e-10
+    assert rel(got, ref) < 1e-10
+    assert info.numops == rinfo["numops"]
+
+
+def test_krylov_expm_real(ctx):
+    """Imaginary-time step on a real state: exp(-tau H) x stays real."""
+    import networksolvers_amd as ns
+    W = models.xxz_mpo(6)
+    mps = chain.random_mps(6, 2, 12, seed=5)
+    L = np.ones((1, 1, 1))
+    for i in range(2):
+        L = chain.env_left(L, mps[i], W[i])
+    R = np.ones((1, 1, 1))
+    for i in range(5, 3, -1):
+        R = chain.env_right(R, mps[i], W[i])
+    th = chain.theta2(mps[2], mps[3])
+    Hd = chain.heff2_dense_matrix(L, W[2], W[3], R)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W[2]), ctx.upload(W[3]), ctx.upload(R))
+    x = ctx.upload(th)
+    ns.krylov_expm(H, x, -0.7)
+    assert x.numpy().dtype == np.float64
+    assert rel(x.numpy().reshape(-1), sl.expm(-0.7 * Hd) @ th.reshape(-1)) < 1e-10
+    with pytest.raises(Exception, match="complex"):
+        ns.krylov_expm(H, x, -0.7j)
+
+
+@pytest.mark.parametrize("m,n", [(8, 12), (40, 40), (96, 70), (130, 260)])
+@pytest.mark.parametrize("rows_trailing", [False, True])
+@pytest.mark.parametrize("cutoff", [1e-14, 1e-6])
+def test_factorize_c128(ctx, m, n, rows_trailing, cutoff):
+    """Complex splits take the Gram route for both "svd" and "eigen": vectors agree
+    element-wise (same phase gauge) where the spectrum resolves them (P >= 1e-6
+    P0), the kept product U C everywhere."""
+    import networksolvers_amd as ns
+    r = min(m, n)
+    U0, _ = np.linalg.qr(crand((m, r), 1))
+    V0, _ = np.linalg.qr(crand((n, r), 2))
+    s = np.exp(-np.arange(r) / 6.0)
+    M = (U0 * s) @ V0.conj().T
+    theta = M.reshape(m // 2, 2, n // 2, 2)
+    U, C, info = ns.factorize(ctx.upload(theta), 2, rows_trailing, cutoff=cutoff, maxdim=64)
+    Uh, Ch = U.numpy(), C.numpy()
+    if not rows_trailing:
+        Uo, Co, io = linalg.factorize(M, cutoff=cutoff, maxdim=64)
+        Ug, Cg = Uh.reshape(m, -1), Ch.reshape(-1, n)
+    else:
+        Uo, Co, io = linalg.factorize(M.T, cutoff=cutoff, maxdim=64)
+        Ug, Cg = Uh.reshape(-1, n).T, Ch.reshape(m, -1).T
+    assert info.kept == io.kept and info.route == io.route
+    assert abs(info.truncerr - io.truncerr) < 1e-13 + 1e-8 * io.truncerr
+    k = info.kept
+    np.testing.assert_allclose(Ug.conj().T @ Ug, np.eye(k), atol=1e-12)
+    np.testing.assert_allclose(Ug @ Cg, Uo @ Co, atol=1e-12)
+    good = io.spectrum[:k] >= 1e-6 * io.spectrum[0]
+    np.testing.assert_allclose(Ug[:, good], Uo[:, good], atol=1e-9)
+    np.testing.assert_allclose(Cg[good], Co[good], atol=1e-9)
+
+
+@pytest.mark.parametrize("m,n", [(6, 10), (32, 32), (80, 50), (33, 140)])
+@pytest.mark.parametrize("rows_trailing", [False, True])
+def test_qr_c128(ctx, m, n, rows_trailing):
+    import networksolvers_amd as ns
+    A = crand((m, 1, n), m + n)
+    Q, R = ns.qr(ctx.upload(A), 1 if rows_trailing else 2, rows_trailing)
+    Qh, Rh = Q.numpy(), R.numpy()
+    if not rows_trailing:
+        Qo, Ro = linalg.qr_pos(A.reshape(m, n))
+        assert rel(Qh.reshape(m, -1), Qo) < 1e-10
+        assert rel(Rh.reshape(-1, n), Ro) < 1e-10
+    else:
+        Qo, Ro = linalg.qr_pos(A.reshape(m, n).T)      # A^T = Q R -> device Q = Qo^T, R = Ro^T
+        assert rel(Qh.reshape(-1, n), Qo.T) < 1e-10
+        assert rel(Rh.reshape(m, -1), Ro.T) < 1e-10
+
+
+# -------------------------------------------------------------------- TDVP
+def _device(ctx, W, mps):
+    import networksolvers_amd as ns
+    return ns.TTNO(ctx, W), ns.TTN.from_arrays(ctx, mps)
+
+
+@pytest.mark.parametrize("nsites,order", [(2, 2), (2, 4), (1, 2), (1, 4)])
+def test_tdvp_exact_at_full_rank(ctx, nsites, order):
+    import networksolvers_amd as ns
+    N = 8
+    W = models.xxz_mpo(N, Jz=0.7, hz=0.1)
+    mps = chain.random_mps(N, 2, 64, seed=3)
+    H, psi = _device(ctx, W, mps)
+    out = ns.tdvp(H, psi, [0.0, 0.05, 0.1], nsites=nsites, tdvp_order=order,
+                  updater_kwargs=dict(solver=ns.exponentiate_solver))
+    want = sl.expm(-1j * chain.mpo_to_dense(W) * 0.1) @ chain.mps_to_dense(mps).reshape(-1)
+    got = chain.mps_to_dense(out.to_arrays()).reshape(-1)
+    assert np.linalg.norm(got - want) < 1e-9
+
+
+@pytest.mark.parametrize("nsites", [1, 2])
+def test_tdvp_truncated_matches_oracle(ctx, nsites):
+    """Neel state, N=12 XXZ, maxdim 12, cutoff 1e-10: the truncating TDVP path of
+    the device follows the oracle's step for step (bond dims equal, fidelity)."""
+    import networksolvers_amd as ns
+    N = 12
+    W = models.xxz_mpo(N, Jz=0.5)
+    if nsites == 2:
+        mps = chain.product_mps([0, 1] * (N // 2), 2)
+    else:   # 1-site TDVP cannot grow bonds: start from a bond-8 state
+        mps = chain.random_mps(N, 2, 8, seed=17)
+    ts = [0.0, 0.1, 0.2, 0.3]
+    trunc = dict(cutoff=1e-10, maxdim=12)
+    ref, _ = otdvp.tdvp_chain(W, mps, ts, nsites=nsites, tdvp_order=2, maxdim=12, cutoff=1e-10)
+    H, psi = _device(ctx, W, mps)
+    out = ns.tdvp(H, psi, ts, nsites=nsites, tdvp_order=2, inserter_kwargs=dict(trunc=trunc),
+                  updater_kwargs=dict(solver=ns.exponentiate_solver))
+    got = out.to_arrays()
+    assert [a.shape for a in got] == [a.shape for a in ref]
+    ov = chain.mps_inner(ref, got)
+    fid = abs(ov) ** 2 / (abs(chain.mps_inner(ref, ref)) * abs(chain.mps_inner(got, got)))
+    assert 1 - fid < 1e-9
+
+
+def test_tdvp_runge_kutta_matches_oracle(ctx):
+    import networksolvers_amd as ns
+    N = 8
+    W = models.heisenberg_mpo(N)
+    mps = chain.random_mps(N, 2, 16, seed=19)
+    ts = [0.0, 0.02, 0.04]
+    ref, _ = otdvp.tdvp_chain(W, mps, ts, nsites=2, tdvp_order=2, solver="rk4")
+    H, psi = _device(ctx, W, mps)
+    out = ns.tdvp(H, psi, ts, nsites=2, tdvp_order=2)      # default solver: runge_kutta_solver, order 4
+    got = chain.mps_to_dense(out.to_arrays()).reshape(-1)
+    assert rel(got, chain.mps_to_dense(ref).reshape(-1)) < 1e-10
+
+
+def test_tdvp_ground_state_phase(ctx):
+    """The reference's regression (test_tree_tdvp.jl:48-77) on a 10-site chain:
+    DMRG ground state, then 1- and 2-site TDVP to t = 0.1 in steps of 0.02; the
+    norm stays > 0.999, overlaps > 0.99, and <psi(t)|gs> has phase E t (1e-4)."""
+    import networksolvers_amd as ns
+    N = 10
+    Hm = ns.heisenberg_mpo(N)
+    H = ns.TTNO(ctx, Hm)
+    psi0 = ns.TTN.from_arrays(ctx, chain.product_mps([1, 0] * (N // 2), 2))
+    E, gs = ns.dmrg(H, psi0, nsweeps=5, nsites=2, inserter_kwargs=dict(trunc=dict(cutoff=1e-10, maxdim=100)))
+    times = [0.0, 0.02, 0.04, 0.06, 0.08, 0.10]
+    kw = dict(inserter_kwargs=dict(trunc=dict(cutoff=1e-10, maxdim=100)))
+    psi1 = ns.tdvp(H, gs, times, nsites=1, **kw)
+    psi2 = ns.tdvp(H, gs, times, nsites=2, **kw)
+    a1, a2, g = psi1.to_arrays(), psi2.to_arrays(), gs.to_arrays()
+    assert np.sqrt(abs(chain.mps_inner(a1, a1))) > 0.999
+    assert np.sqrt(abs(chain.mps_inner(a2, a2))) > 0.999
+    assert abs(chain.mps_inner(a1, g)) > 0.99
+    assert abs(chain.mps_inner(a1, a2)) > 0.99
+    z = chain.mps_inner(a1, g)
+    assert abs(np.arctan(z.imag / z.real) - E * 0.1) < 1e-4
+
+
+def test_applyexp_imaginary_time(ctx):
+    """applyexp with real exponents (imaginary time, normalising inserter) on a real
+    state: the state stays float64, follows the oracle's evolution (energy to
+    1e-10, same bond dimensions), and its energy drops toward, never below, the
+    ground energy (ED fixture)."""
+    import networksolvers_amd as ns
+    N = 10
+    Hm = ns.heisenberg_mpo(N)
+    H = ns.TTNO(ctx, Hm)
+    mps = chain.random_mps(N, 2, 8, seed=23)
+    psi0 = ns.TTN.from_arrays(ctx, mps)
+    e_start = ns.expect_mpo(psi0, H)
+    exps = [-0.5 * k for k in range(1, 21)]
+    out = ns.applyexp(H, psi0, exps, nsites=2, tdvp_order=2, updater_kwargs=dict(solver=ns.exponentiate_solver),
+                      inserter_kwargs=dict(normalize=True, trunc=dict(cutoff=1e-10, maxdim=32)))
+    assert out.dtype == np.float64
+    ref, _ = otdvp.applyexp_chain(Hm, mps, exps, nsites=2, tdvp_order=2, maxdim=32, cutoff=1e-10, normalize=True)
+    assert out.linkdims() == [a.shape[2] for a in ref[:-1]]
+    e = ns.expect_mpo(out, H)
+    e_ref = chain.mpo_expectation(Hm, ref)
+    assert abs(e - e_ref) < 1e-10
+    e0 = _ed("heisenberg_s12_chain", "10")
+    assert e0 - 1e-10 < e < e_start - 1.0
