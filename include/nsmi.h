@@ -25,6 +25,9 @@
  *   extracter.jl:14, applyexp.jl:38 itn.position (environments)   -> nsmi_env_update
  *   extracter.jl:6, applyexp.jl:37  orthogonalize / qr            -> nsmi_qr
  *   inserter.jl:23                  it.factorize(theta, inds; trunc...) -> nsmi_factorize
+ *   operator_map.jl:3-10 and ProjTTN on trees (up to four environments per
+ *   region, multi-neighbour environment updates)  -> nsmi_tensordot,
+ *                                   nsmi_permute, nsmi_netop_create
  */
 #ifndef NSMI_H
 #define NSMI_H
@@ -183,6 +186,41 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
  * extracter.jl:6).  Same row/transpose conventions as nsmi_factorize. */
 int nsmi_qr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Q,
             nsmi_tensor** R);
+
+/* ------------------------------------------------- general tree contractions */
+/* out = A with its axes permuted: out.dims[k] = A.dims[perm[k]] (rank <= 8).
+ * Index relabelling of tree tensors (ITensors' named-index contraction order,
+ * extracter.jl:7, inserter.jl:23-29). */
+int nsmi_permute(nsmi_ctx* ctx, const nsmi_tensor* A, const int* perm, nsmi_tensor** out);
+
+/* out[free(A)..., free(B)...] = sum over the n axis pairs (axes_a[i], axes_b[i])
+ * of op(A) op(B), op = conj when conj_* != 0 (C128); free axes keep their order.
+ * A and B share one element type.  One MFMA GEMM (operands permuted into matrix
+ * form when needed).  Replaces the ITensor contractions of the tree environment
+ * update (ProjTTN position, extracter.jl:14) and theta (extracter.jl:7). */
+int nsmi_tensordot(nsmi_ctx* ctx, const nsmi_tensor* A, int conj_a, const nsmi_tensor* B, int conj_b, int n,
+                   const int* axes_a, const int* axes_b, nsmi_tensor** out);
+
+/* One step of a contraction program: running = tensordot(running, op(operand),
+ * axes_x, axes_o) with op = conj when conj != 0. */
+typedef struct nsmi_net_step {
+  int operand;    /* index into the operand list            */
+  int n;          /* number of contracted axis pairs (<= 8) */
+  int axes_x[8];  /* axes of the running tensor             */
+  int axes_o[8];  /* axes of the operand                    */
+  int conj;
+} nsmi_net_step;
+
+/* An effective Hamiltonian given as a contraction program (optimal_map on a tree
+ * region, operator_map.jl:3-10, with up to four incident environments): applying
+ * it runs the steps on x and permutes the result by out_perm, which must give x's
+ * shape.  The returned handle is used like nsmi_heff_create's (apply, rayleigh,
+ * Lanczos, Krylov exponential, destroy); FP64 / C128 only.  Operands must stay
+ * alive until nsmi_heff_destroy; they are read in [contracted..., free...] order,
+ * permuted once here when needed. */
+int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_steps,
+                      const nsmi_net_step* steps, int x_rank, const int64_t* x_dims, const int* out_perm,
+                      nsmi_heff** out);
 
 /* ------------------------------------------------------------- profiling */
 int nsmi_profile_enable(nsmi_ctx* ctx, int on);

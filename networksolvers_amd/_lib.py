@@ -45,6 +45,11 @@ class ProfileStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("total_ms", C.c_double), ("flops", C.c_double), ("bytes", C.c_double)]
 
 
+class NetStep(C.Structure):
+    _fields_ = [("operand", C.c_int), ("n", C.c_int), ("axes_x", C.c_int * 8), ("axes_o", C.c_int * 8),
+                ("conj", C.c_int)]
+
+
 class NsmiError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
@@ -87,6 +92,11 @@ SIGNATURES = {
                                    C.POINTER(SolverInfo)]),
     "nsmi_factorize": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.POINTER(Trunc), _PP, _PP, C.POINTER(TruncInfo)]),
     "nsmi_qr": (C.c_int, [_VP, _VP, C.c_int, C.c_int, _PP, _PP]),
+    "nsmi_permute": (C.c_int, [_VP, _VP, C.POINTER(C.c_int), _PP]),
+    "nsmi_tensordot": (C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), _PP]),
+    "nsmi_netop_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.c_int, C.POINTER(NetStep), C.c_int,
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int), _PP]),
     "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),
     "nsmi_profile_reset": (C.c_int, [_VP]),
     "nsmi_profile_query": (C.c_int, [_VP, C.c_int, C.POINTER(ProfileStats)]),

@@ -19,11 +19,11 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import kernels
+from . import kernels, tree
 from .network import ProjTTN, TTN, TTNO, _absorb_left, _absorb_right
 from .region_plans import tdvp_regions
-from .sweep import (current_region, next_region, operator, optimal_map, region_plan, state, sweep_iterator,
-                    sweep_solve, updater)
+from .sweep import (current_region, next_region, operator, optimal_map, projected_operator, region_plan, state,
+                    sweep_iterator, sweep_solve, updater)
 
 
 @dataclass
@@ -148,7 +148,10 @@ def _(T: ApplyExpProblem, local_state, region_iter, *, nsites, time_step, solver
         curr, nxt = current_region(region_iter), next_region(region_iter)
         if nxt is not None and list(nxt) != list(curr):
             (v,) = curr
-            local_state = _back_evolve_bond(T, v, nxt[0] > v, local_state, time_step, solver, precision, kws)
+            if isinstance(state(T), tree.TreeTTN):
+                local_state = tree.back_evolve_bond(T, v, list(nxt), local_state, time_step, solver, kws)
+            else:
+                local_state = _back_evolve_bond(T, v, nxt[0] > v, local_state, time_step, solver, precision, kws)
     T.current_time = T.current_time + time_step
     if outputlevel >= 2:
         print(f"  Region {current_region(region_iter)}: time step {time_step}")
@@ -181,7 +184,9 @@ def applyexp_problem(init_prob: ApplyExpProblem, exponents, *, extracter_kwargs=
     return prob
 
 
-def _as_complex(psi: TTN) -> TTN:
+def _as_complex(psi):
+    if isinstance(psi, tree.TreeTTN):
+        return tree.TreeTTN.from_arrays(psi.ctx, psi.tg, psi.to_arrays(), dtype=np.complex128)
     return TTN.from_arrays(psi.ctx, psi.to_arrays(), psi.graph, dtype=np.complex128)
 
 
@@ -191,7 +196,7 @@ def applyexp(H: TTNO, init_state: TTN, exponents, *, env_precision="fp64", **kws
     cplx = any(isinstance(z, complex) and z.imag != 0 for z in exponents) or np.iscomplexobj(np.asarray(exponents))
     psi = _as_complex(init_state) if (cplx and init_state.dtype != np.complex128) else init_state.copy()
     psi.ortho_region = init_state.ortho_region
-    init_prob = ApplyExpProblem(state=psi, operator=ProjTTN(H, env_precision))
+    init_prob = ApplyExpProblem(state=psi, operator=projected_operator(H, env_precision))
     return state(applyexp_problem(init_prob, exponents, **kws))
 
 

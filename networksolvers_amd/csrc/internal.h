@@ -212,6 +212,15 @@ void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int row
 void zqr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_trailing, nsmi_tensor** Qout,
          nsmi_tensor** Rout);
 
+// general network contractions (network.hip): a contraction program executed as
+// an H_eff (tree regions)
+struct NetProgram;
+void net_apply(nsmi_ctx* ctx, NetProgram* p, const double* x, double* y);
+void net_release(nsmi_ctx* ctx, NetProgram* p);
+int64_t net_size(const NetProgram* p);
+bool net_is_complex(const NetProgram* p);
+nsmi_heff* heff_wrap_net(NetProgram* p);  // region.hip: an nsmi_heff that runs the program
+
 // H_eff operator access for the Krylov solvers (krylov.cpp)
 void heff_apply_dev(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y);
 int64_t heff_size(const nsmi_heff* op);

@@ -1,0 +1,424 @@
+// General tensor-network contractions for regions of tree tensor networks.
+//
+// On a tree a region's effective Hamiltonian and the environment updates join up
+// to four environments, the region's MPO tensors and the state (operator_map.jl:3-10
+// with ITensorNetworks' ProjTTN on a tree).  They are expressed with two device
+// primitives:
+//   permute    out[i_perm...] = in[i...]   (rank <= 8, F64 or interleaved C128)
+//   tensordot  C[free(A)..., free(B)...] = sum op(A) op(B) over paired axes, run as
+//              one MFMA GEMM; an operand is permuted into matrix form first unless
+//              its axes already read as a row-major (or transposed) matrix.
+// A network operator ("netop") is a fixed contraction program: the running tensor
+// starts as the Krylov vector x and is contracted step by step with stored operands
+// (environments and MPO tensors), then permuted back to x's layout.  It is an
+// nsmi_heff, so the Lanczos eigensolver and the Krylov exponential drive it
+// unchanged.  Operands are put in [contracted..., free...] order once, at creation.
+#include <algorithm>
+#include <numeric>
+
+#include "internal.h"
+
+namespace nsmi {
+namespace {
+
+constexpr int kMaxRank = 8;
+
+struct PermArgs {
+  int rank;
+  int64_t odims[kMaxRank];
+  int64_t istr[kMaxRank];  // input stride of the input axis that lands on output axis k
+  int64_t n;
+};
+
+struct alignas(16) c16 {
+  double re, im;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute(const T* __restrict__ in, T* __restrict__ out, PermArgs a) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < a.n; t += stride) {
+    int64_t r = t, off = 0;
+    for (int k = a.rank - 1; k >= 0; --k) {
+      const int64_t ik = r % a.odims[k];
+      r /= a.odims[k];
+      off += ik * a.istr[k];
+    }
+    out[t] = in[off];
+  }
+}
+
+int64_t numel_of(int rank, const int64_t* dims) {
+  int64_t n = 1;
+  for (int i = 0; i < rank; ++i) n *= dims[i];
+  return n;
+}
+
+bool is_identity(int rank, const int* perm) {
+  for (int i = 0; i < rank; ++i)
+    if (perm[i] != i) return false;
+  return true;
+}
+
+void check_perm(int rank, const int* perm) {
+  require(rank >= 0 && rank <= kMaxRank, "permute: rank above 8 is not served");
+  bool seen[kMaxRank] = {};
+  for (int i = 0; i < rank; ++i) {
+    require(perm[i] >= 0 && perm[i] < rank && !seen[perm[i]], "permute: not a permutation");
+    seen[perm[i]] = true;
+  }
+}
+
+// out = permute(in) with out.dims[k] = in.dims[perm[k]]
+void permute_dev(nsmi_ctx* ctx, size_t elt, const void* in, int rank, const int64_t* idims, const int* perm,
+                 void* out) {
+  check_perm(rank, perm);
+  PermArgs a{};
+  a.rank = rank;
+  int64_t istride[kMaxRank];
+  int64_t s = 1;
+  for (int i = rank - 1; i >= 0; --i) {
+    istride[i] = s;
+    s *= idims[i];
+  }
+  for (int k = 0; k < rank; ++k) {
+    a.odims[k] = idims[perm[k]];
+    a.istr[k] = istride[perm[k]];
+  }
+  a.n = numel_of(rank, idims);
+  if (a.n == 0) return;
+  const unsigned blocks = unsigned(std::min<int64_t>((a.n + 255) / 256, int64_t(1) << 20));
+  if (elt == 16)
+    hipLaunchKernelGGL(k_permute<c16>, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<const c16*>(in),
+                       static_cast<c16*>(out), a);
+  else
+    hipLaunchKernelGGL(k_permute<double>, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<const double*>(in),
+                       static_cast<double*>(out), a);
+  NSMI_LAUNCH_CHECK();
+}
+
+GemmDesc mdesc(int64_t M, int64_t N, int64_t K, bool ta, bool tb) {
+  GemmDesc g{};
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.transA = ta;
+  g.transB = tb;
+  g.lda = ta ? M : K;
+  g.ldb = tb ? K : N;
+  g.ldc = N;
+  g.batch = 1;
+  return g;
+}
+
+// How a tensor is read as a matrix for a contraction over `axes`:
+//   front = false: rows = free axes, cols = contracted axes (the A side, M x K)
+//   front = true:  rows = contracted axes, cols = free axes (the B side, K x N)
+struct MatView {
+  int perm[kMaxRank];  // permutation to the wanted order
+  bool direct = false; // already in the wanted order
+  bool trans = false;  // in the opposite order: use the transposed GEMM form
+  int nfree = 0;
+  int free_axes[kMaxRank];
+  int64_t rows = 1, cols = 1;
+};
+
+MatView plan_view(int rank, const int64_t* dims, int n, const int* axes, bool front) {
+  MatView v;
+  bool contracted[kMaxRank] = {};
+  for (int i = 0; i < n; ++i) {
+    require(axes[i] >= 0 && axes[i] < rank && !contracted[axes[i]], "tensordot: bad or repeated axis");
+    contracted[axes[i]] = true;
+  }
+  for (int i = 0; i < rank; ++i)
+    if (!contracted[i]) v.free_axes[v.nfree++] = i;
+  int64_t kprod = 1, fprod = 1;
+  for (int i = 0; i < n; ++i) kprod *= dims[axes[i]];
+  for (int i = 0; i < v.nfree; ++i) fprod *= dims[v.free_axes[i]];
+  int want[kMaxRank], opp[kMaxRank];
+  int w = 0, o = 0;
+  if (!front) {
+    for (int i = 0; i < v.nfree; ++i) want[w++] = v.free_axes[i];
+    for (int i = 0; i < n; ++i) want[w++] = axes[i];
+    for (int i = 0; i < n; ++i) opp[o++] = axes[i];
+    for (int i = 0; i < v.nfree; ++i) opp[o++] = v.free_axes[i];
+    v.rows = fprod;
+    v.cols = kprod;
+  } else {
+    for (int i = 0; i < n; ++i) want[w++] = axes[i];
+    for (int i = 0; i < v.nfree; ++i) want[w++] = v.free_axes[i];
+    for (int i = 0; i < v.nfree; ++i) opp[o++] = v.free_axes[i];
+    for (int i = 0; i < n; ++i) opp[o++] = axes[i];
+    v.rows = kprod;
+    v.cols = fprod;
+  }
+  std::copy(want, want + rank, v.perm);
+  v.direct = is_identity(rank, want);
+  v.trans = !v.direct && is_identity(rank, opp);
+  return v;
+}
+
+void gemm_any(nsmi_ctx* ctx, bool cplx, const GemmDesc& g, const void* A, bool conjA, const void* B, bool conjB,
+              void* C, int role) {
+  if (g.M == 0 || g.N == 0) return;
+  if (cplx)
+    gemm_c128(ctx, g, static_cast<const double*>(A), conjA, static_cast<const double*>(B), conjB,
+              static_cast<double*>(C), role);
+  else
+    gemm_f64(ctx, g, 1.0, static_cast<const double*>(A), static_cast<const double*>(B), 0.0, static_cast<double*>(C),
+             role);
+}
+
+void permuted_dims(int rank, const int64_t* dims, const int* perm, int64_t* out) {
+  for (int k = 0; k < rank; ++k) out[k] = dims[perm[k]];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ network operator
+struct NetStepPlan {
+  const void* opd = nullptr;  // operand as a K x N matrix ([contracted..., free...])
+  void* owned = nullptr;      // pool block when the operand had to be permuted
+  bool conj = false;
+  MatView xv;                 // how the running tensor is read (M x K)
+  int rank_in = 0;
+  int64_t dims_in[kMaxRank] = {};
+  int rank_out = 0;
+  int64_t dims_out[kMaxRank] = {};
+  int64_t M = 0, K = 0, N = 0;
+};
+
+struct NetProgram {
+  bool cplx = false;
+  size_t elt = 8;
+  std::vector<NetStepPlan> steps;
+  int out_perm[kMaxRank] = {};
+  bool out_identity = true;
+  int rank_x = 0;
+  int64_t dims_x[kMaxRank] = {};
+  int64_t n = 0;
+  void* buf[2] = {nullptr, nullptr};
+  void* pbuf = nullptr;
+  double flops = 0.0;
+};
+
+void net_release(nsmi_ctx* ctx, NetProgram* p) {
+  if (!p) return;
+  for (auto& s : p->steps) scratch_release(ctx, s.owned);
+  scratch_release(ctx, p->buf[0]);
+  scratch_release(ctx, p->buf[1]);
+  scratch_release(ctx, p->pbuf);
+  delete p;
+}
+
+void net_apply(nsmi_ctx* ctx, NetProgram* p, const double* x, double* y) {
+  ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, p->flops, 0.0);
+  const void* cur = x;
+  const int ns = int(p->steps.size());
+  for (int i = 0; i < ns; ++i) {
+    const NetStepPlan& s = p->steps[i];
+    const void* a = cur;
+    bool ta = false;
+    if (s.xv.direct) {
+    } else if (s.xv.trans) {
+      ta = true;
+    } else {
+      permute_dev(ctx, p->elt, cur, s.rank_in, s.dims_in, s.xv.perm, p->pbuf);
+      a = p->pbuf;
+    }
+    void* out = (i == ns - 1 && p->out_identity) ? static_cast<void*>(y) : p->buf[i & 1];
+    gemm_any(ctx, p->cplx, mdesc(s.M, s.N, s.K, ta, false), a, false, s.opd, s.conj, out, kRoleHeff2);
+    cur = out;
+  }
+  if (ns == 0 || !p->out_identity) {
+    const int rank = ns ? p->steps.back().rank_out : p->rank_x;
+    const int64_t* dims = ns ? p->steps.back().dims_out : p->dims_x;
+    permute_dev(ctx, p->elt, cur, rank, dims, p->out_perm, y);
+  }
+}
+
+int64_t net_size(const NetProgram* p) { return p->n; }
+bool net_is_complex(const NetProgram* p) { return p->cplx; }
+
+}  // namespace nsmi
+
+using namespace nsmi;
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                             \
+  }                                                               \
+  catch (const nsmi::Error& e) {                                  \
+    if (ctxp) (ctxp)->err = e.what();                             \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what(); \
+    return NSMI_ERR_INVALID;                                      \
+  }                                                               \
+  return NSMI_OK;
+
+extern "C" {
+
+int nsmi_permute(nsmi_ctx* ctx, const nsmi_tensor* A, const int* perm, nsmi_tensor** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *out = nullptr;
+  require(A && perm, "permute: A and perm are required");
+  check_perm(A->rank, perm);
+  int64_t od[kMaxRank];
+  permuted_dims(A->rank, A->dims, perm, od);
+  nsmi_tensor* t = tensor_new(ctx, A->dtype, A->rank, od);
+  try {
+    if (is_identity(A->rank, perm))
+      NSMI_HIP(hipMemcpyAsync(t->data, A->data, A->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    else
+      permute_dev(ctx, dtype_size(A->dtype), A->data, A->rank, A->dims, perm, t->data);
+  } catch (...) {
+    tensor_delete(ctx, t);
+    throw;
+  }
+  *out = t;
+  API_END(ctx)
+}
+
+int nsmi_tensordot(nsmi_ctx* ctx, const nsmi_tensor* A, int conj_a, const nsmi_tensor* B, int conj_b, int n,
+                   const int* axes_a, const int* axes_b, nsmi_tensor** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *out = nullptr;
+  require(A && B, "tensordot: A and B are required");
+  require(A->dtype == B->dtype, "tensordot: A and B must have the same element type");
+  require(n >= 0 && n <= kMaxRank && (n == 0 || (axes_a && axes_b)), "tensordot: bad axis count");
+  for (int i = 0; i < n; ++i) {
+    require(axes_a[i] >= 0 && axes_a[i] < A->rank && axes_b[i] >= 0 && axes_b[i] < B->rank,
+            "tensordot: axis out of range");
+    require(A->dims[axes_a[i]] == B->dims[axes_b[i]], "tensordot: contracted dimensions differ");
+  }
+  const bool cplx = A->dtype == NSMI_C128;
+  const size_t elt = dtype_size(A->dtype);
+  MatView va = plan_view(A->rank, A->dims, n, axes_a, false);
+  MatView vb = plan_view(B->rank, B->dims, n, axes_b, true);
+  require(va.nfree + vb.nfree <= kMaxRank, "tensordot: result rank above 8 is not served");
+  int64_t od[kMaxRank];
+  int r = 0;
+  for (int i = 0; i < va.nfree; ++i) od[r++] = A->dims[va.free_axes[i]];
+  for (int i = 0; i < vb.nfree; ++i) od[r++] = B->dims[vb.free_axes[i]];
+  nsmi_tensor* t = tensor_new(ctx, A->dtype, r, od);
+  try {
+    const void* pa = A->data;
+    const void* pb = B->data;
+    Scratch sa(ctx, (va.direct || va.trans) ? 0 : A->bytes);
+    Scratch sb(ctx, (vb.direct || vb.trans) ? 0 : B->bytes);
+    if (!va.direct && !va.trans) {
+      permute_dev(ctx, elt, A->data, A->rank, A->dims, va.perm, sa.p);
+      pa = sa.p;
+    }
+    if (!vb.direct && !vb.trans) {
+      permute_dev(ctx, elt, B->data, B->rank, B->dims, vb.perm, sb.p);
+      pb = sb.p;
+    }
+    const int64_t K = va.cols;
+    const double f = (cplx ? 8.0 : 2.0) * double(va.rows) * double(vb.cols) * double(K);
+    ProfScope ps(ctx, NSMI_K_ENV, f, double(elt) * double(A->numel + B->numel + t->numel));
+    gemm_any(ctx, cplx, mdesc(va.rows, vb.cols, K, va.trans, vb.trans), pa, conj_a != 0, pb, conj_b != 0, t->data,
+             kRoleEnv);
+  } catch (...) {
+    tensor_delete(ctx, t);
+    throw;
+  }
+  *out = t;
+  API_END(ctx)
+}
+
+int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_steps,
+                      const nsmi_net_step* steps, int x_rank, const int64_t* x_dims, const int* out_perm,
+                      nsmi_heff** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  NetProgram* p = nullptr;
+  API_BEGIN
+  *out = nullptr;
+  require(n_operands >= 0 && n_steps >= 0 && (n_steps == 0 || (operands && steps)), "netop: bad operand/step list");
+  require(x_rank >= 1 && x_rank <= kMaxRank && x_dims && out_perm, "netop: bad x description");
+  p = new NetProgram();
+  p->rank_x = x_rank;
+  std::copy(x_dims, x_dims + x_rank, p->dims_x);
+  p->n = numel_of(x_rank, x_dims);
+  int dtype = -1;
+  for (int i = 0; i < n_operands; ++i) {
+    require(operands[i] != nullptr, "netop: null operand");
+    if (dtype < 0) dtype = operands[i]->dtype;
+    require(operands[i]->dtype == dtype, "netop: operands must share one element type");
+  }
+  if (dtype < 0) dtype = NSMI_F64;
+  p->cplx = dtype == NSMI_C128;
+  p->elt = dtype_size(dtype);
+  int rank = x_rank;
+  int64_t dims[kMaxRank];
+  std::copy(x_dims, x_dims + x_rank, dims);
+  size_t max_buf = 0, max_perm = 0;
+  for (int i = 0; i < n_steps; ++i) {
+    const nsmi_net_step& st = steps[i];
+    require(st.operand >= 0 && st.operand < n_operands, "netop: operand index out of range");
+    require(st.n >= 0 && st.n <= kMaxRank, "netop: bad contracted-axis count");
+    const nsmi_tensor* O = operands[st.operand];
+    for (int k = 0; k < st.n; ++k) {
+      require(st.axes_x[k] >= 0 && st.axes_x[k] < rank && st.axes_o[k] >= 0 && st.axes_o[k] < O->rank,
+              "netop: axis out of range");
+      require(dims[st.axes_x[k]] == O->dims[st.axes_o[k]], "netop: contracted dimensions differ");
+    }
+    NetStepPlan s;
+    s.rank_in = rank;
+    std::copy(dims, dims + rank, s.dims_in);
+    s.xv = plan_view(rank, dims, st.n, st.axes_x, false);
+    MatView ov = plan_view(O->rank, O->dims, st.n, st.axes_o, true);
+    require(s.xv.nfree + ov.nfree <= kMaxRank, "netop: intermediate rank above 8 is not served");
+    s.M = s.xv.rows;
+    s.K = s.xv.cols;
+    s.N = ov.cols;
+    s.conj = st.conj != 0 && p->cplx;
+    if (ov.direct) {
+      s.opd = O->data;
+    } else {
+      s.owned = scratch(ctx, O->bytes);
+      permute_dev(ctx, p->elt, O->data, O->rank, O->dims, ov.perm, s.owned);
+      s.opd = s.owned;
+    }
+    int r = 0;
+    for (int k = 0; k < s.xv.nfree; ++k) s.dims_out[r++] = dims[s.xv.free_axes[k]];
+    for (int k = 0; k < ov.nfree; ++k) s.dims_out[r++] = O->dims[ov.free_axes[k]];
+    s.rank_out = r;
+    if (!s.xv.direct && !s.xv.trans) max_perm = std::max(max_perm, size_t(numel_of(rank, dims)));
+    max_buf = std::max(max_buf, size_t(numel_of(r, s.dims_out)));
+    p->flops += (p->cplx ? 8.0 : 2.0) * double(s.M) * double(s.N) * double(s.K);
+    rank = r;
+    std::copy(s.dims_out, s.dims_out + r, dims);
+    p->steps.push_back(s);
+  }
+  require(rank == x_rank, "netop: the program does not return to the rank of x");
+  check_perm(rank, out_perm);
+  std::copy(out_perm, out_perm + rank, p->out_perm);
+  for (int k = 0; k < rank; ++k)
+    require(dims[out_perm[k]] == x_dims[k], "netop: the program does not map x's space to itself");
+  p->out_identity = is_identity(rank, out_perm);
+  if (max_buf) {
+    p->buf[0] = scratch(ctx, p->elt * max_buf);
+    p->buf[1] = scratch(ctx, p->elt * max_buf);
+  }
+  if (max_perm) p->pbuf = scratch(ctx, p->elt * max_perm);
+  *out = heff_wrap_net(p);
+  p = nullptr;
+  }
+  catch (const nsmi::Error& e) {
+    net_release(ctx, p);
+    ctx->err = e.what();
+    return e.code;
+  }
+  catch (const std::exception& e) {
+    net_release(ctx, p);
+    ctx->err = std::string("internal: ") + e.what();
+    return NSMI_ERR_INVALID;
+  }
+  return NSMI_OK;
+}
+
+}  // extern "C"

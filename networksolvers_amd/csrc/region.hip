@@ -35,6 +35,7 @@ struct nsmi_heff {
   int64_t Kp1 = 0, Kp2 = 0;
   double flops_gemm1 = 0, flops_gemm2 = 0, flops_mix = 0;
   int64_t n = 0;  // elements of theta
+  nsmi::NetProgram* net = nullptr;  // tree regions: a general contraction program instead of the fixed form
 };
 
 namespace nsmi {
@@ -318,6 +319,10 @@ void small_sym_eigh(int n, const double* Ain, double* w, double* V) {
 
 // ------------------------------------------------------------------ H_eff
 static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y, int prec) {
+  if (op->net) {
+    net_apply(ctx, op->net, x, y);
+    return;
+  }
   const int64_t M1 = op->cl * op->D0, K1 = op->cl, N1 = op->dphys * op->cr;
   const int64_t M2 = op->cl * op->dphys, K2 = op->D2 * op->cr, N2 = op->cr;
   GemmDesc g1 = desc(M1, N1, K1, false, false);
@@ -402,8 +407,21 @@ static void heff_release(nsmi_ctx* ctx, nsmi_heff* op) {
   void* bufs[] = {op->Mx, op->T1, op->T3, op->L32, op->R32, op->Lhi, op->Llo, op->Rhi, op->Rlo,
                   op->xThi, op->xTlo, op->T3hi, op->T3lo, op->T1f};
   for (void* b : bufs) scratch_release(ctx, b);
+  net_release(ctx, op->net);
   delete op;
 }
+
+namespace nsmi {
+nsmi_heff* heff_wrap_net(NetProgram* p) {
+  nsmi_heff* op = new nsmi_heff();
+  op->net = p;
+  op->nsite = 0;
+  op->precision = NSMI_PREC_FP64;
+  op->cplx = net_is_complex(p);
+  op->n = net_size(p);
+  return op;
+}
+}  // namespace nsmi
 
 extern "C" {
 

@@ -34,21 +34,32 @@ def env_update(E: DeviceTensor, A: DeviceTensor, W: DeviceTensor, side: int, pre
     return DeviceTensor(ctx, h)
 
 
-class HeffMap:
-    """Device effective Hamiltonian of a 1- or 2-site region (operator_map.jl:3-10).
+def permute(A: DeviceTensor, perm) -> DeviceTensor:
+    """out.dims[k] = A.dims[perm[k]] (index relabelling for tree tensors)."""
+    ctx = A.ctx
+    p = (C.c_int * max(len(perm), 1))(*perm)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.nsmi_permute(ctx.h, A.h, p, C.byref(h)))
+    return DeviceTensor(ctx, h)
 
-    Callable like the reference's ``psi -> optimal_map(P, psi)``: ``H(x)`` returns
-    a new device tensor.  Keeps its environment and MPO tensors alive."""
 
-    def __init__(self, L: DeviceTensor, W1: DeviceTensor, W2: DeviceTensor | None, R: DeviceTensor,
-                 precision="fp64"):
-        self.ctx = L.ctx
-        self._keep = (L, W1, W2, R)
-        self.precision = precision
-        h = C.c_void_p()
-        self.ctx.check(self.ctx.lib.nsmi_heff_create(self.ctx.h, L.h, W1.h, W2.h if W2 is not None else None,
-                                                     R.h, _prec(precision), C.byref(h)))
-        self.h = h
+def tensordot(A: DeviceTensor, B: DeviceTensor, axes_a, axes_b, conj_a=False, conj_b=False) -> DeviceTensor:
+    """numpy.tensordot(op(A), op(B), (axes_a, axes_b)) on the device (one MFMA GEMM)."""
+    ctx = A.ctx
+    n = len(axes_a)
+    if len(axes_b) != n:
+        raise ValueError("tensordot: axis lists differ in length")
+    aa = (C.c_int * max(n, 1))(*axes_a)
+    ab = (C.c_int * max(n, 1))(*axes_b)
+    h = C.c_void_p()
+    ctx.check(ctx.lib.nsmi_tensordot(ctx.h, A.h, int(bool(conj_a)), B.h, int(bool(conj_b)), n, aa, ab, C.byref(h)))
+    return DeviceTensor(ctx, h)
+
+
+class _OperatorBase:
+    """Shared handle logic of the device effective Hamiltonians."""
+    ctx: Context
+    h: C.c_void_p | None = None
 
     def close(self):
         if self.h is not None and not self.ctx.closed:
@@ -73,6 +84,56 @@ class HeffMap:
         e = C.c_double()
         self.ctx.check(self.ctx.lib.nsmi_heff_rayleigh(self.ctx.h, self.h, x.h, C.byref(e)))
         return e.value
+
+
+class HeffMap(_OperatorBase):
+    """Device effective Hamiltonian of a 1- or 2-site chain region (operator_map.jl:3-10).
+
+    Callable like the reference's ``psi -> optimal_map(P, psi)``: ``H(x)`` returns
+    a new device tensor.  Keeps its environment and MPO tensors alive."""
+
+    def __init__(self, L: DeviceTensor, W1: DeviceTensor, W2: DeviceTensor | None, R: DeviceTensor,
+                 precision="fp64"):
+        self.ctx = L.ctx
+        self._keep = (L, W1, W2, R)
+        self.precision = precision
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.nsmi_heff_create(self.ctx.h, L.h, W1.h, W2.h if W2 is not None else None,
+                                                     R.h, _prec(precision), C.byref(h)))
+        self.h = h
+
+
+class NetMap(_OperatorBase):
+    """Effective Hamiltonian of a tree region as a device contraction program
+    (operator_map.jl:3-10 with up to four incident environments).
+
+    ``steps`` are (operand index, axes of the running tensor, axes of the operand,
+    conj) tuples; after them the running tensor is permuted by ``out_perm`` back
+    to the layout of x (shape ``x_shape``).  Keeps its operands alive."""
+
+    def __init__(self, operands, steps, x_shape, out_perm):
+        self.ctx = operands[0].ctx if operands else None
+        if self.ctx is None:
+            raise ValueError("NetMap needs at least one operand")
+        self._keep = tuple(operands)
+        self.precision = "fp64"
+        ops = (C.c_void_p * len(operands))(*[o.h.value for o in operands])
+        st = (_lib.NetStep * max(len(steps), 1))()
+        for i, (j, ax, ao, conj) in enumerate(steps):
+            if len(ax) != len(ao) or len(ax) > 8:
+                raise ValueError("NetMap: bad contraction step")
+            st[i].operand = j
+            st[i].n = len(ax)
+            for k, (a, b) in enumerate(zip(ax, ao)):
+                st[i].axes_x[k] = a
+                st[i].axes_o[k] = b
+            st[i].conj = int(bool(conj))
+        dims = (C.c_int64 * len(x_shape))(*x_shape)
+        perm = (C.c_int * len(out_perm))(*out_perm)
+        h = C.c_void_p()
+        self.ctx.check(self.ctx.lib.nsmi_netop_create(self.ctx.h, len(operands), ops, len(steps), st, len(x_shape),
+                                                      dims, perm, C.byref(h)))
+        self.h = h
 
 
 @dataclass
@@ -141,5 +202,5 @@ def qr(A: DeviceTensor, n_row_dims: int, rows_trailing: bool):
     return DeviceTensor(ctx, Q), DeviceTensor(ctx, R)
 
 
-__all__ = ["Context", "DeviceTensor", "theta", "env_update", "HeffMap", "lanczos_eig", "krylov_expm", "factorize",
-           "qr"]
+__all__ = ["Context", "DeviceTensor", "theta", "env_update", "HeffMap", "NetMap", "lanczos_eig", "krylov_expm",
+           "factorize", "qr", "permute", "tensordot"]

@@ -208,13 +208,36 @@ class ProjTTN:
 
 
 def expect_mpo(psi: TTN, H: TTNO, precision="fp64") -> float:
-    """<psi|H|psi> / <psi|psi> by a full left-to-right environment pass in FP64."""
+    """<psi|H|psi> / <psi|psi> by a full left-to-right environment pass in FP64
+    (on a tree: the Rayleigh quotient at an orthogonality centre)."""
+    if not isinstance(psi, TTN):
+        from . import tree
+        return tree.expect_mpo(psi, H)
     ctx = H.ctx
     E = ctx.upload(np.ones((1, 1, 1), dtype=psi.dtype))
     for i in range(len(psi)):
         E = kernels.env_update(E, psi[i], H[i], 0, precision)
     val = E.numpy().reshape(-1)[0]
     return float(np.real(val)) / norm2(psi)
+
+
+def expect_local(psi, op) -> dict:
+    """<psi|op_v|psi> / <psi|psi> for every vertex v of a chain or tree state
+    (ITensorNetworks ``expect`` with a one-site operator): the orthogonality
+    centre of a copy walks site by site and the value is read at the centre."""
+    if not isinstance(psi, TTN):
+        from . import tree
+        return tree.expect_local(psi, op)
+    c = psi.copy()
+    opd = psi.ctx.upload(np.ascontiguousarray(np.asarray(op), dtype=psi.dtype))
+    out = {}
+    for v in range(len(c)):
+        orthogonalize(c, [v])
+        A = c[v]
+        B = kernels.tensordot(A, opd, [1], [1])                 # [l, r, s']
+        B = kernels.permute(B, [0, 2, 1])                       # [l, s', r]
+        out[v] = float(np.real(A.inner(B) / A.inner(A)))
+    return out
 
 
 def norm2(psi: TTN) -> float:

@@ -24,7 +24,7 @@ import functools
 import itertools
 from dataclasses import dataclass, field
 
-from . import kernels
+from . import kernels, tree
 from .network import ProjTTN, TTN, TTNO, orthogonalize
 from .region_plans import euler_sweep, truncation_parameters
 
@@ -188,6 +188,15 @@ def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=Non
     tensors, (subspace expansion hook), then ``position`` builds the environments."""
     truncation_parameters(sweep, **(trunc or {}))
     region = current_region(region_iter)
+    if subspace_algorithm is not None:
+        raise NotImplementedError(
+            "Subspace expansion (subspace_expand!) not defined for requested combination of "
+            f"subspace_algorithm={subspace_algorithm!r} and problem types")
+    if isinstance(state(problem), tree.TreeTTN):
+        psi = state(problem)
+        local_state = tree.extract(psi, list(region))
+        problem.operator = operator(problem).position(psi, list(region))
+        return problem, local_state
     psi = orthogonalize(state(problem), region)
     problem.state = psi
     lo, hi = min(region), max(region)
@@ -199,17 +208,19 @@ def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=Non
         local_state = psi[lo].copy()
     else:
         raise ValueError(f"Region of length {len(region)} not currently supported")
-    if subspace_algorithm is not None:
-        raise NotImplementedError(
-            "Subspace expansion (subspace_expand!) not defined for requested combination of "
-            f"subspace_algorithm={subspace_algorithm!r} and problem types")
     problem.operator = operator(problem).position(psi, region)
     return problem, local_state
 
 
 def optimal_map(P: ProjTTN, psi: TTN, precision="fp64") -> kernels.HeffMap:
     """operator_map.jl:3-10: the region's effective Hamiltonian as a device map
-    (environments of the incident edges and the region's MPO tensors)."""
+    (environments of the incident edges and the region's MPO tensors; on a tree a
+    contraction program over up to four environments)."""
+    if isinstance(P, tree.TreeProjTTN):
+        if precision != "fp64":
+            raise ValueError("tree regions are served in FP64 (ComplexF64) only")
+        region = P.sites()
+        return P.netmap(psi, tree.region_labels(psi, region), tree.region_shape(psi, region))
     L, R = P.left_right(psi)
     sites = P.sites()
     H = P.operator()
@@ -257,18 +268,23 @@ def inserter(problem, local_tensor, region_iter, *, normalize=False, set_orthogo
     region = current_region(region_iter)
     psi = state(problem)
     info = None
-    if len(region) == 1:
-        C = local_tensor
-    elif len(region) == 2:
-        u, v = region
-        lo = min(u, v)
-        U, C, info = kernels.factorize(local_tensor, 2, u != lo, cutoff=t["cutoff"], maxdim=t["maxdim"],
-                                       mindim=t["mindim"], which_decomp=which_decomp)
-        psi[u] = U
+    if isinstance(psi, tree.TreeTTN):
+        if len(region) > 2:
+            raise ValueError(f"Region of length {len(region)} not currently supported")
+        info = tree.insert(psi, list(region), local_tensor, t, which_decomp)
     else:
-        raise ValueError(f"Region of length {len(region)} not currently supported")
+        if len(region) == 1:
+            C = local_tensor
+        elif len(region) == 2:
+            u, v = region
+            lo = min(u, v)
+            U, C, info = kernels.factorize(local_tensor, 2, u != lo, cutoff=t["cutoff"], maxdim=t["maxdim"],
+                                           mindim=t["mindim"], which_decomp=which_decomp)
+            psi[u] = U
+        else:
+            raise ValueError(f"Region of length {len(region)} not currently supported")
+        psi[region[-1]] = C
     v = region[-1]
-    psi[v] = C
     if set_orthogonal_region:
         psi.ortho_region = [v]
     if normalize:
@@ -302,8 +318,15 @@ def eigsolve_problem(init_prob: EigsolveProblem, *, nsweeps, nsites=1, outputlev
 def eigsolve(H: TTNO, init_state: TTN, *, env_precision="fp64", **kws):
     """eigsolve.jl:69-74: wraps (H, psi0) into an EigsolveProblem.  The state is
     copied; ``env_precision`` selects the environment-update arithmetic."""
-    init_prob = EigsolveProblem(state=init_state.copy(), operator=ProjTTN(H, env_precision))
+    init_prob = EigsolveProblem(state=init_state.copy(), operator=projected_operator(H, env_precision))
     return eigsolve_problem(init_prob, **kws)
+
+
+def projected_operator(H, env_precision="fp64"):
+    """ProjTTN of a chain MPO or a tree TTN-MPO (eigsolve.jl:71, applyexp.jl:86)."""
+    if isinstance(H, tree.TreeTTNO):
+        return tree.TreeProjTTN(H, env_precision)
+    return ProjTTN(H, env_precision)
 
 
 def dmrg(*args, **kws):

@@ -1,0 +1,609 @@
+"""Tree tensor networks on the device: state (TTN), operator (TTN-MPO), projected
+operator and the region step on a tree, for the same sweep drivers as the chain.
+
+Restates the ITensorNetworks pieces the reference sweeps over a general tree
+(``ttn``, ``ProjTTN``, ``orthogonalize``, ``position``, ``inner``/``expect``):
+regions of one or two vertices with up to four incident environments
+(operator_map.jl:3-10), multi-neighbour environment updates (extracter.jl:14),
+theta over the shared link (extracter.jl:7) and the truncated factorize with U on
+first(region) (inserter.jl:23-29).
+
+Layout (row-major): the tensor of vertex v is A[l_n1, ..., l_nk, s] with one
+link per neighbour in the graph's adjacency order (``compute_adjacencies``), the
+MPO tensor W[w_n1, ..., w_nk, s', s], and the environment of a directed edge
+p -> q is E[bra, mpo, ket] (as on the chain).  Contractions are driven by index
+labels: ("k", e) / ("b", e) ket / bra link of edge e, ("w", e) MPO link, ("s", v)
+/ ("t", v) ket / bra site.  Every contraction is a device tensordot (one MFMA
+GEMM); the region's H_eff is a device contraction program (``kernels.NetMap``).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import kernels
+from .device import Context, DeviceTensor
+from .region_plans import Graph, compute_adjacencies
+
+_BRA = {"k": "b", "s": "t", "q": "r"}
+
+
+def _bra(label):
+    return (_BRA.get(label[0], label[0]),) + tuple(label[1:])
+
+
+def _tdot(A, la, B, lb, conj_a=False, conj_b=False):
+    common = [lab for lab in la if lab in lb]
+    out = kernels.tensordot(A, B, [la.index(x) for x in common], [lb.index(x) for x in common], conj_a, conj_b)
+    return out, [x for x in la if x not in common] + [x for x in lb if x not in common]
+
+
+def _program(x_labels, x_shape, operands):
+    """A device contraction program from labels: ``operands`` is a list of
+    (tensor, labels) contracted in order with the running tensor (starting as x)
+    over their common labels; the result must carry the bra labels of x and is
+    permuted back to x's layout."""
+    cur, steps = list(x_labels), []
+    for j, (_, lt) in enumerate(operands):
+        common = [lab for lab in cur if lab in lt]
+        steps.append((j, [cur.index(x) for x in common], [lt.index(x) for x in common], False))
+        cur = [x for x in cur if x not in common] + [x for x in lt if x not in common]
+    target = [_bra(x) for x in x_labels]
+    if sorted(map(repr, cur)) != sorted(map(repr, target)):
+        raise RuntimeError(f"contraction program does not close: {cur} vs {target}")
+    return kernels.NetMap([t for t, _ in operands], steps, list(x_shape), [cur.index(t) for t in target])
+
+
+def _permute_to(A, la, target):
+    if sorted(map(repr, la)) != sorted(map(repr, target)):
+        raise ValueError(f"label sets differ: {la} vs {target}")
+    perm = [la.index(x) for x in target]
+    if perm == list(range(len(perm))):
+        return A
+    return kernels.permute(A, perm)
+
+
+class TreeGraph:
+    """A tree with a fixed vertex order, adjacency order and edge keys."""
+
+    def __init__(self, graph: Graph):
+        self.graph = graph
+        self.vertices = list(graph.vertices)
+        self.index = {v: i for i, v in enumerate(self.vertices)}
+        self.adj = compute_adjacencies(graph)
+        if len(graph.edges) != len(self.vertices) - 1:
+            raise ValueError("not a tree: |E| != |V| - 1")
+        self._side = {}
+
+    def key(self, u, v):
+        return (u, v) if self.index[u] < self.index[v] else (v, u)
+
+    def neighbors(self, v):
+        return self.adj[v]
+
+    def side(self, u, v):
+        """Vertices on u's side of the edge u - v."""
+        k = (u, v)
+        if k not in self._side:
+            seen, stack = {u}, [u]
+            while stack:
+                x = stack.pop()
+                for y in self.adj[x]:
+                    if y not in seen and not (x == u and y == v):
+                        seen.add(y)
+                        stack.append(y)
+            self._side[k] = seen
+        return self._side[k]
+
+    def path(self, a, b):
+        """Vertices on the tree path a -> b (inclusive)."""
+        prev, stack = {a: None}, [a]
+        while stack:
+            x = stack.pop()
+            if x == b:
+                break
+            for y in self.adj[x]:
+                if y not in prev:
+                    prev[y] = x
+                    stack.append(y)
+        out = [b]
+        while out[-1] != a:
+            out.append(prev[out[-1]])
+        return out[::-1]
+
+    def site_labels(self, v):
+        return [("k", self.key(v, n)) for n in self.adj[v]] + [("s", v)]
+
+    def mpo_labels(self, v):
+        return [("w", self.key(v, n)) for n in self.adj[v]] + [("t", v), ("s", v)]
+
+
+# ------------------------------------------------------------------ state / operator
+class TreeTTN:
+    """Tree tensor-network state on the device (ITensorNetworks ``ttn``), one
+    tensor per vertex in the layout above.  ``ortho_region`` is None while the
+    gauge is unknown; writes bump per-vertex stamps checked by the environments."""
+
+    def __init__(self, ctx: Context, graph, tensors: dict, ortho_region=None):
+        self.ctx = ctx
+        self.tg = graph if isinstance(graph, TreeGraph) else TreeGraph(graph)
+        self.graph = self.tg.graph
+        self.tensors = {v: (t if isinstance(t, DeviceTensor) else ctx.upload(t)) for v, t in tensors.items()}
+        if set(self.tensors) != set(self.tg.vertices):
+            raise ValueError("one tensor per vertex is required")
+        for v in self.tg.vertices:
+            if len(self.tensors[v].shape) != len(self.tg.adj[v]) + 1:
+                raise ValueError(f"tensor of vertex {v!r} must have one link per neighbour plus the site index")
+        self.ortho_region = ortho_region
+        self.clock = 0
+        self.stamp = {v: 0 for v in self.tg.vertices}
+
+    @classmethod
+    def from_arrays(cls, ctx, graph, arrays: dict, dtype=None):
+        if dtype is None:
+            dtype = np.complex128 if any(np.iscomplexobj(a) for a in arrays.values()) else np.float64
+        return cls(ctx, graph, {v: np.ascontiguousarray(a, dtype=dtype) for v, a in arrays.items()})
+
+    @property
+    def dtype(self):
+        return next(iter(self.tensors.values())).dtype
+
+    def vertices(self):
+        return list(self.tg.vertices)
+
+    def __len__(self):
+        return len(self.tensors)
+
+    def __getitem__(self, v):
+        return self.tensors[v]
+
+    def __setitem__(self, v, t: DeviceTensor):
+        self.tensors[v] = t
+        self.clock += 1
+        self.stamp[v] = self.clock
+
+    def linkdim(self, u, v):
+        return self.tensors[u].shape[self.tg.adj[u].index(v)]
+
+    def linkdims(self):
+        return {tuple(e): self.linkdim(e[0], e[1]) for e in self.graph.edges}
+
+    def maxlinkdim(self):
+        return max(self.linkdims().values(), default=1)
+
+    def to_arrays(self):
+        return {v: t.numpy() for v, t in self.tensors.items()}
+
+    def copy(self):
+        return TreeTTN(self.ctx, self.tg, {v: t.copy() for v, t in self.tensors.items()}, self.ortho_region)
+
+
+class TreeTTNO:
+    """Tree MPO (TTN operator) on the device, W[w links..., s', s] per vertex; a
+    copy per element type is kept so that contractions with complex states stay
+    in one type."""
+
+    def __init__(self, ctx: Context, graph, tensors: dict):
+        self.ctx = ctx
+        self.tg = graph if isinstance(graph, TreeGraph) else TreeGraph(graph)
+        self.graph = self.tg.graph
+        self.host = {v: np.ascontiguousarray(t, dtype=np.float64) for v, t in tensors.items()}
+        self._dev = {}
+
+    def __len__(self):
+        return len(self.host)
+
+    def tensor(self, v, dtype=np.float64):
+        key = (v, np.dtype(dtype).str)
+        if key not in self._dev:
+            self._dev[key] = self.ctx.upload(self.host[v].astype(dtype))
+        return self._dev[key]
+
+
+def tree_mpo(graph, d, onsite=None, bond_terms=None):
+    """TTN-MPO of H = sum_v onsite(v) + sum_{edges (u, v)} sum_a c_a O_a(u) P_a(v).
+
+    ``onsite``: d x d matrix for every vertex, or dict vertex -> matrix, or None.
+    ``bond_terms``: list of (c, O, P) for every edge (u, v) of the edge list, or a
+    dict edge -> list, or a callable edge -> list.  The operator is built rooted at
+    the last vertex: every edge (child -> parent) carries D = m + 2 states (0:
+    identity so far, 1..m: term a opened in the child's subtree at the child, m+1:
+    complete), as the finite-state construction the reference delegates to
+    ``itn.ttn(::OpSum)`` (test/dmrg/test_tree_dmrg.jl:44-51).  Returns a dict
+    vertex -> W[w links in adjacency order..., s', s] (numpy, float64)."""
+    tg = graph if isinstance(graph, TreeGraph) else TreeGraph(graph)
+    G = tg.graph
+    eye = np.eye(d)
+
+    def terms_of(e):
+        if bond_terms is None:
+            return []
+        if callable(bond_terms):
+            return list(bond_terms(e))
+        if isinstance(bond_terms, dict):
+            return list(bond_terms.get(tuple(e), bond_terms.get(tuple(e[::-1]), [])))
+        return list(bond_terms)
+
+    def onsite_of(v):
+        if onsite is None:
+            return np.zeros((d, d))
+        if isinstance(onsite, dict):
+            return np.asarray(onsite.get(v, np.zeros((d, d))), dtype=float)
+        return np.asarray(onsite, dtype=float)
+
+    edge_terms = {}
+    m = 0
+    for (a, b) in G.edges:
+        t = terms_of((a, b))
+        edge_terms[tg.key(a, b)] = (a, b, t)
+        m = max(m, len(t))
+    D = m + 2
+    root = tg.vertices[-1]
+    parent = {root: None}
+    order = [root]
+    for x in order:
+        for y in tg.adj[x]:
+            if y not in parent:
+                parent[y] = x
+                order.append(y)
+    W = {}
+    for v in tg.vertices:
+        nb = tg.adj[v]
+        shape = [D] * len(nb) + [d, d]
+        T = np.zeros(shape)
+        p = parent[v]
+        children = [n for n in nb if n != p]
+        pos = {n: i for i, n in enumerate(nb)}
+
+        def put(out_state, child_states, op):
+            idx = [0] * len(nb)
+            for c, st in child_states.items():
+                idx[pos[c]] = st
+            if p is not None:
+                idx[pos[p]] = out_state
+            elif out_state != D - 1:
+                return
+            T[tuple(idx)] += op
+
+        put(0, {}, eye)
+        if p is not None:
+            src, dst, terms = edge_terms[tg.key(v, p)]
+            for a, (c, O, P) in enumerate(terms, start=1):
+                put(a, {}, np.asarray(O if v == src else P, dtype=float))
+        put(D - 1, {}, onsite_of(v))
+        for ch in children:
+            put(D - 1, {ch: D - 1}, eye)
+            src, dst, terms = edge_terms[tg.key(v, ch)]
+            for a, (c, O, P) in enumerate(terms, start=1):
+                put(D - 1, {ch: a}, c * np.asarray(P if v == dst else O, dtype=float))
+        W[v] = T
+    return W
+
+
+def _pauli():
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.array([[1.0, 0.0], [0.0, -1.0]])
+    return X, Z
+
+
+def tfim_tree_mpo(graph, h=1.0, J=1.0):
+    """H = -J sum_<ij> Z_i Z_j - h sum_i X_i (Pauli matrices), D = 3 per edge."""
+    X, Z = _pauli()
+    return tree_mpo(graph, 2, onsite=-h * X, bond_terms=[(-J, Z, Z)])
+
+
+def heisenberg_tree_mpo(graph, d=2, J=1.0, edges=None):
+    """H = J sum S_i.S_j over the graph's edges (or only over ``edges``), D = 5."""
+    S = (d - 1) / 2
+    m = np.arange(S, -S - 1, -1)
+    Sz = np.diag(m)
+    Sp = np.zeros((d, d))
+    for i in range(1, d):
+        Sp[i - 1, i] = np.sqrt(S * (S + 1) - m[i] * (m[i] + 1))
+    Sm = Sp.T
+    terms = [(J, Sz, Sz), (J / 2, Sp, Sm), (J / 2, Sm, Sp)]
+    if edges is None:
+        return tree_mpo(graph, d, bond_terms=terms)
+    keep = {tuple(e) for e in edges} | {tuple(e[::-1]) for e in edges}
+    return tree_mpo(graph, d, bond_terms=lambda e: terms if tuple(e) in keep else [])
+
+
+def product_ttn(ctx, graph, states: dict, d, dtype=np.float64) -> TreeTTN:
+    """Product state with vertex v in basis state states[v] (all links of dimension 1)."""
+    tg = graph if isinstance(graph, TreeGraph) else TreeGraph(graph)
+    arrays = {}
+    for v in tg.vertices:
+        a = np.zeros([1] * len(tg.adj[v]) + [d], dtype=dtype)
+        a[(0,) * len(tg.adj[v]) + (states[v],)] = 1.0
+        arrays[v] = a
+    return TreeTTN(ctx, tg, arrays)
+
+
+def random_ttn_arrays(graph, d, chi, seed, dtype=np.float64):
+    """Seeded random tree tensors: link of edge (u, v) of dimension
+    min(chi, d^|u side|, d^|v side|), entries i.i.d. N(0,1) from numpy PCG64(seed)
+    drawn vertex by vertex in vertex order (real then imaginary part).  Not
+    normalised (``random_ttn`` normalises on the device)."""
+    tg = graph if isinstance(graph, TreeGraph) else TreeGraph(graph)
+    rng = np.random.default_rng(seed)
+    arrays = {}
+    for v in tg.vertices:
+        dims = []
+        for n in tg.adj[v]:
+            a, b = len(tg.side(v, n)), len(tg.side(n, v))
+            dims.append(int(min(chi, d ** min(a, 60), d ** min(b, 60))))
+        shape = dims + [d]
+        x = rng.standard_normal(shape)
+        if np.issubdtype(np.dtype(dtype), np.complexfloating):
+            x = x + 1j * rng.standard_normal(shape)
+        arrays[v] = x.astype(dtype)
+    return arrays
+
+
+def random_ttn(ctx, graph, d, chi, seed, dtype=np.float64) -> TreeTTN:
+    psi = TreeTTN(ctx, graph, random_ttn_arrays(graph, d, chi, seed, dtype))
+    r = psi.tg.vertices[-1]
+    orthogonalize(psi, [r])
+    psi[r] = psi[r].copy().scale_(1.0 / psi[r].norm())
+    return psi
+
+
+# ------------------------------------------------------------------- gauge
+def _qr_toward(psi: TreeTTN, x, p):
+    """psi[x] = Q (isometry toward p), R absorbed into psi[p]."""
+    tg = psi.tg
+    e = tg.key(x, p)
+    lx = tg.site_labels(x)
+    rows = [lab for lab in lx if lab != ("k", e)]
+    A = _permute_to(psi[x], lx, rows + [("k", e)])
+    Q, R = kernels.qr(A, len(rows), False)                     # Q[rows..., m], R[m, link]
+    psi[x] = _permute_to(Q, rows + [("k", e)], lx)
+    lp = tg.site_labels(p)
+    B, lb = _tdot(R, [("n", e), ("k", e)], psi[p], lp)
+    psi[p] = _permute_to(B, [("k", e) if lab == ("n", e) else lab for lab in lb], lp)
+
+
+def orthogonalize(psi: TreeTTN, region) -> TreeTTN:
+    """Make the region the orthogonality centre (extracter.jl:6 -> ITensorNetworks
+    ``orthogonalize``): with an unknown gauge every vertex outside the region is
+    made an isometry toward it, leaves first; with a known centre only the path
+    from it to the region is QR-walked."""
+    tg = psi.tg
+    region = list(region)
+    rs = set(region)
+    cur = psi.ortho_region
+    if cur is not None and set(cur) <= rs:
+        psi.ortho_region = region
+        return psi
+    if cur is None:
+        parent, order = {}, []
+        frontier = list(region)
+        seen = set(region)
+        while frontier:
+            nxt = []
+            for x in frontier:
+                for y in tg.adj[x]:
+                    if y not in seen:
+                        seen.add(y)
+                        parent[y] = x
+                        order.append(y)
+                        nxt.append(y)
+            frontier = nxt
+        for x in reversed(order):
+            _qr_toward(psi, x, parent[x])
+    else:
+        cur = list(cur)
+        target = min(region, key=lambda r: min(len(tg.path(c, r)) for c in cur))
+        c0 = min(cur, key=lambda c: len(tg.path(c, target)))
+        for c in cur:                                          # collapse a multi-vertex centre onto c0
+            if c != c0:
+                path = tg.path(c, c0)
+                for a, b in zip(path[:-1], path[1:]):
+                    _qr_toward(psi, a, b)
+        path = tg.path(c0, target)
+        for a, b in zip(path[:-1], path[1:]):
+            if a in rs:
+                break
+            _qr_toward(psi, a, b)
+    psi.ortho_region = region
+    return psi
+
+
+# ------------------------------------------------------------ projected operator
+class TreeProjTTN:
+    """Projected tree operator with environments cached per directed edge (u, v)
+    = u's side seen from v (ITensorNetworks ``ProjTTN``).  An environment is
+    rebuilt when any vertex on its side was written after it was built."""
+
+    def __init__(self, H: TreeTTNO, env_precision="fp64"):
+        if env_precision != "fp64":
+            raise ValueError("tree environments are computed in FP64")
+        self.H = H
+        self.ctx = H.ctx
+        self.tg = H.tg
+        self.pos = list(self.tg.vertices)
+        self.envs = {}
+        self.builds = 0
+        self.env_precision = env_precision
+
+    def operator(self):
+        return self.H
+
+    def sites(self):
+        return list(self.pos)
+
+    def incident_edges(self):
+        rs = set(self.pos)
+        return [(n, r) for r in self.pos for n in self.tg.adj[r] if n not in rs]
+
+    def environment(self, psi: TreeTTN, e):
+        u, v = e
+        ent = self.envs.get(e)
+        if ent is not None:
+            t, built = ent
+            if all(psi.stamp[x] <= built for x in self.tg.side(u, v)):
+                return t
+        t = self._build(psi, u, v, psi[u])
+        self.builds += 1
+        self.envs[e] = (t, psi.clock)
+        return t
+
+    def _build(self, psi, u, v, Au, incoming=None):
+        """E_{u->v} from the incoming environments of u (all neighbours but v),
+        W_u and the ket/bra tensor Au."""
+        tg = self.tg
+        ev = tg.key(u, v)
+        lu = tg.site_labels(u)
+        ins = [(n, self.environment(psi, (n, u)) if incoming is None else incoming[n])
+               for n in tg.adj[u] if n != v]
+        ops = []
+        for n, E in ins:
+            e = tg.key(n, u)
+            ops.append((E, [("b", e), ("w", e), ("k", e)], False))
+        Wl = tg.mpo_labels(u)
+        ops.insert(1 if ops else 0, (self.H.tensor(u, psi.dtype), Wl, False))
+        ops.append((Au, [_bra(lab) for lab in lu], True))
+        X, lx = Au, list(lu)
+        for T, lt, conj in ops:
+            X, lx = _tdot(X, lx, T, lt, False, conj)
+        return _permute_to(X, lx, [("b", ev), ("w", ev), ("k", ev)])
+
+    def position(self, psi: TreeTTN, region) -> "TreeProjTTN":
+        self.pos = list(region)
+        for e in self.incident_edges():
+            self.environment(psi, e)
+        return self
+
+    def netmap(self, psi: TreeTTN, x_labels, x_shape) -> kernels.NetMap:
+        """The region's effective Hamiltonian as a device contraction program:
+        per region vertex its first incident environment, its MPO tensor, then the
+        rest, so the running tensor carries at most two open MPO links."""
+        tg = self.tg
+        operands = []
+        rs = set(self.pos)
+        for r in self.pos:
+            envs = []
+            for n in tg.adj[r]:
+                if n in rs:
+                    continue
+                e = tg.key(n, r)
+                envs.append((self.environment(psi, (n, r)), [("b", e), ("w", e), ("k", e)]))
+            operands += envs[:1] + [(self.H.tensor(r, psi.dtype), tg.mpo_labels(r))] + envs[1:]
+        return _program(x_labels, x_shape, operands)
+
+
+# ----------------------------------------------------------------- region step
+def region_labels(psi: TreeTTN, region):
+    """Labels of the region's local tensor (theta over the shared link for two
+    vertices, the vertex tensor for one)."""
+    tg = psi.tg
+    if len(region) == 1:
+        return tg.site_labels(region[0])
+    u, v = region
+    e = tg.key(u, v)
+    return [x for x in tg.site_labels(u) if x != ("k", e)] + [x for x in tg.site_labels(v) if x != ("k", e)]
+
+
+def region_shape(psi: TreeTTN, region):
+    tg = psi.tg
+    dims = {}
+    for r in region:
+        for lab, n in zip(tg.site_labels(r), list(psi[r].shape)):
+            dims[lab] = n
+    return [dims[lab] for lab in region_labels(psi, region)]
+
+
+def extract(psi: TreeTTN, region):
+    """extracter.jl:6-7 on a tree: orthogonalize, then theta = psi[u] psi[v]."""
+    orthogonalize(psi, region)
+    tg = psi.tg
+    if len(region) == 1:
+        return psi[region[0]].copy()
+    if len(region) != 2:
+        raise ValueError(f"Region of length {len(region)} not currently supported")
+    u, v = region
+    if v not in tg.adj[u]:
+        raise ValueError(f"2-site region {region} is not an edge")
+    th, _ = _tdot(psi[u], tg.site_labels(u), psi[v], tg.site_labels(v))
+    return th
+
+
+def insert(psi: TreeTTN, region, local, trunc, which_decomp=None):
+    """inserter.jl:16-29 on a tree: U on first(region) (its labels but the shared
+    link), C on last(region); returns the factorize info (None for 1 vertex)."""
+    tg = psi.tg
+    if len(region) == 1:
+        psi[region[0]] = local
+        return None
+    u, v = region
+    e = tg.key(u, v)
+    lu = [x for x in tg.site_labels(u) if x != ("k", e)]
+    lv = [x for x in tg.site_labels(v) if x != ("k", e)]
+    U, C, info = kernels.factorize(local, len(lu), False, cutoff=trunc["cutoff"], maxdim=trunc["maxdim"],
+                                   mindim=trunc["mindim"], which_decomp=which_decomp)
+    psi[u] = _permute_to(U, lu + [("k", e)], tg.site_labels(u))
+    psi[v] = _permute_to(C, [("k", e)] + lv, tg.site_labels(v))
+    return info
+
+
+def back_evolve_bond(T, v, nxt_region, x, time_step, solver, kws):
+    """applyexp.jl:30-42 on a tree: the first edge (v, w) on the path to the next
+    region; x = Q R over it, R evolved by -time_step with the bond H_eff (the
+    environment of v's side rebuilt with Q, w's side as cached); returns Q R(t)."""
+    psi, P = T.state, T.operator
+    tg = psi.tg
+    w = tg.path(v, nxt_region[0])[1]
+    e = tg.key(v, w)
+    lv = tg.site_labels(v)
+    rows = [lab for lab in lv if lab != ("k", e)]
+    A = _permute_to(x, lv, rows + [("k", e)])
+    Q, R = kernels.qr(A, len(rows), False)                     # Q[rows..., m], R[m, link]
+    Qc = _permute_to(Q, rows + [("k", e)], lv)
+    Eq = P._build(psi, v, w, Qc)                               # v's side with Q: [b, w, k] over the new link
+    Ew = P.environment(psi, (w, v))                            # w's side: [b, w, k] over the old link
+    q = ("q", e)
+    op = _program([("k", q), ("k", e)], R.shape,
+                  [(Eq, [("b", q), ("w", e), ("k", q)]), (Ew, [("b", e), ("w", e), ("k", e)])])
+    try:
+        Rt, _info = solver(op, -time_step, R, **kws)
+    finally:
+        op.close()
+    out, lo = _tdot(Qc, [("k", q) if lab == ("k", e) else lab for lab in lv], Rt, [("k", q), ("k", e)])
+    return _permute_to(out, lo, lv)
+
+
+# ---------------------------------------------------------------- observables
+def expect_mpo(psi: TreeTTN, H: TreeTTNO) -> float:
+    """<psi|H|psi> / <psi|psi>: orthogonalize a copy to the last vertex, then the
+    1-vertex effective Hamiltonian's Rayleigh quotient there."""
+    c = psi.copy()
+    r = c.tg.vertices[-1]
+    orthogonalize(c, [r])
+    P = TreeProjTTN(H)
+    P.position(c, [r])
+    op = P.netmap(c, c.tg.site_labels(r), c[r].shape)
+    try:
+        return op.rayleigh(c[r])
+    finally:
+        op.close()
+
+
+def expect_local(psi: TreeTTN, op) -> dict:
+    """<psi|op_v|psi> / <psi|psi> for every vertex v (ITensorNetworks ``expect``
+    with a one-site operator, as in the reference's quench example): the
+    orthogonality centre of a copy walks vertex by vertex, and at the centre the
+    value is <A_v, op A_v> / <A_v, A_v>."""
+    c = psi.copy()
+    opd = psi.ctx.upload(np.ascontiguousarray(np.asarray(op), dtype=psi.dtype))
+    out = {}
+    for v in c.tg.vertices:
+        orthogonalize(c, [v])
+        A = c[v]
+        B = kernels.tensordot(A, opd, [len(A.shape) - 1], [1])   # [..., s'] = sum_s A[..., s] op[s', s]
+        val = A.inner(B) / A.inner(A)
+        out[v] = float(np.real(val))
+    return out
+
+
+__all__ = ["TreeGraph", "TreeTTN", "TreeTTNO", "TreeProjTTN", "tree_mpo", "tfim_tree_mpo", "heisenberg_tree_mpo",
+           "product_ttn", "random_ttn", "random_ttn_arrays", "orthogonalize", "extract", "insert",
+           "back_evolve_bond", "expect_mpo", "expect_local", "region_labels", "region_shape"]

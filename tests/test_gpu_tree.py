@@ -1,0 +1,306 @@
+"""GPU parity of the tree path (general contractions, tree H_eff programs, tree
+DMRG and TDVP) against numpy and the oracle.
+
+Tolerances: permute is exact; tensordot and the H_eff / environment programs
+agree to 1e-12 relative (summation order only); tree DMRG energies per region
+to 1e-10 relative with identical kept bond dimensions; TDVP states to fidelity
+1 - 1e-9 against the oracle and 1e-9 against expm at full bond rank.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import tree as otree
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import networksolvers_amd as ns
+    c = ns.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def build_tree(nbranch=3, nbranch_sites=3):
+    V = [(0, 0)] + [(b, s) for b in range(1, nbranch + 1) for s in range(1, nbranch_sites + 1)]
+    E = []
+    for b in range(1, nbranch + 1):
+        E.append(((0, 0), (b, 1)))
+        for s in range(2, nbranch_sites + 1):
+            E.append(((b, s - 1), (b, s)))
+    return V, E
+
+
+def binary_tree(depth):
+    n = 2 ** (depth + 1) - 1
+    return list(range(n)), [((i - 1) // 2, i) for i in range(1, n)]
+
+
+def chain_plus_ancilla(nchain):
+    V = list(range(1, nchain + 1)) + [0]
+    E = [(j, j + 1) for j in range(1, nchain)] + [(0, nchain // 2)]
+    return V, E
+
+
+def neel(T, V):
+    psi = {}
+    for j, v in enumerate(V, start=1):
+        k = len(T.adj[v])
+        a = np.zeros([1] * k + [2])
+        a[(0,) * k + (0 if j % 2 == 0 else 1,)] = 1.0
+        psi[v] = a
+    return psi
+
+
+def _ed(model, key):
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")) as f:
+        return json.load(f)[model][key]
+
+
+def full_rank_random(T, seed, cplx=False):
+    rng = np.random.default_rng(seed)
+    psi = {}
+    for v in T.vertices:
+        dims = [min(2 ** len(T.side(v, n)), 2 ** len(T.side(n, v))) for n in T.adj[v]]
+        a = rng.standard_normal(dims + [2])
+        if cplx:
+            a = a + 1j * rng.standard_normal(dims + [2])
+        psi[v] = a
+    nrm = np.linalg.norm(otree.ttn_to_dense(T, psi))
+    return {k: a / nrm ** (1 / len(T.vertices)) for k, a in psi.items()}
+
+
+# -------------------------------------------------------------- primitives
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+@pytest.mark.parametrize("shape,perm", [((3, 4, 5), (2, 0, 1)), ((2, 3, 4, 5, 6), (4, 2, 0, 3, 1)),
+                                        ((7, 1, 9), (0, 2, 1)), ((2, 2, 2, 2, 2, 2, 2, 2), (7, 6, 5, 4, 3, 2, 1, 0)),
+                                        ((64, 33), (1, 0))])
+def test_permute(ctx, dtype, shape, perm):
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal(shape)
+    if dtype == np.complex128:
+        A = A + 1j * rng.standard_normal(shape)
+    got = ns.permute(ctx.upload(A.astype(dtype)), perm).numpy()
+    np.testing.assert_array_equal(got, np.transpose(A, perm))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+@pytest.mark.parametrize("sa,sb,aa,ab", [((4, 5, 6), (6, 7), [2], [0]), ((4, 5, 6), (7, 4), [0], [1]),
+                                         ((3, 4, 5, 6), (5, 3, 8), [2, 0], [0, 1]), ((30, 40), (40, 50), [1], [0]),
+                                         ((3, 4), (5, 6), [], []), ((9, 70, 11), (11, 70, 2), [1, 2], [1, 0]),
+                                         ((130, 2, 129), (129, 2, 131), [2], [0])])
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+def test_tensordot(ctx, dtype, sa, sb, aa, ab, conj):
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(2)
+    A, B = rng.standard_normal(sa), rng.standard_normal(sb)
+    if dtype == np.complex128:
+        A = A + 1j * rng.standard_normal(sa)
+        B = B + 1j * rng.standard_normal(sb)
+    got = ns.tensordot(ctx.upload(A.astype(dtype)), ctx.upload(B.astype(dtype)), aa, ab, *conj).numpy()
+    want = np.tensordot(A.conj() if conj[0] else A, B.conj() if conj[1] else B, axes=(aa, ab))
+    assert rel(got, want) < 1e-13
+
+
+def test_netmap_errors(ctx):
+    import networksolvers_amd as ns
+    A = ctx.upload(np.ones((3, 4)))
+    with pytest.raises(Exception, match="netop"):
+        ns.NetMap([A], [(0, [0], [1], False)], [3, 5], [0, 1])     # 3 vs 4 contracted
+    with pytest.raises(Exception, match="netop"):
+        ns.NetMap([A], [(0, [1], [0], False)], [2, 3], [1, 0])     # does not return to x's space
+
+
+# ------------------------------------------------------------- tree region
+def _device_tree(ctx, V, E, W, psi, cplx=False):
+    import networksolvers_amd as ns
+    G = ns.Graph(V, E)
+    H = ns.TreeTTNO(ctx, G, W)
+    dt = np.complex128 if cplx else np.float64
+    return G, H, ns.TreeTTN.from_arrays(ctx, G, psi, dtype=dt)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_tree_env_and_heff_vs_oracle(ctx, cplx):
+    """Environments of every directed edge and the H_eff of every 1- and 2-vertex
+    region on the 7-vertex binary tree (degree-3 vertices: up to four incident
+    environments) against the oracle's contractions."""
+    from networksolvers_amd import tree as ptree
+    V, E = binary_tree(2)
+    T = otree.Tree(V, E)
+    G, H, psi = _device_tree(ctx, V, E, ptree.tfim_tree_mpo(__import__("networksolvers_amd").Graph(V, E), h=0.8),
+                             full_rank_random(T, 9, cplx), cplx)
+    W = H.host
+    arrays = psi.to_arrays()
+    oenv = otree.TreeEnvs(T, W)
+    P = ptree.TreeProjTTN(H)
+    for (a, b) in E:
+        for (u, v) in ((a, b), (b, a)):
+            assert rel(P.environment(psi, (u, v)).numpy(), oenv.get(arrays, u, v)) < 1e-12
+    rng = np.random.default_rng(3)
+    for region in [[0], [1], [3], [1, 0], [0, 2], [3, 1], [2, 6]]:
+        P.position(psi, region)
+        shape = ptree.region_shape(psi, region)
+        x = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+        op = P.netmap(psi, ptree.region_labels(psi, region), shape)
+        got = op(ctx.upload(x)).numpy()
+        envs_in = {(n, r): oenv.get(arrays, n, r) for r in region for n in T.adj[r] if n not in region}
+        want = otree.heff_apply(T, W, envs_in, region, x)
+        assert rel(got, want) < 1e-12, region
+        op.close()
+
+
+def test_tree_dmrg_reference_case(ctx):
+    """test_tree_dmrg.jl:39-76 on the device: |E - E_ED| < 1e-5 with cutoff 1e-5,
+    maxdim 40, 5 sweeps; and every region equal to the oracle (eigenvalue 1e-10
+    relative, kept dimension exact)."""
+    import networksolvers_amd as ns
+    V, E = build_tree()
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.heisenberg_tree_mpo(G)
+    ex = _ed("tree", "build_tree_3_3_heisenberg_sz0")
+    psi0 = neel(T, V)
+    H = ns.TreeTTNO(ctx, G, W)
+    En, psi = ns.dmrg(H, ns.TreeTTN.from_arrays(ctx, G, psi0), nsweeps=5, nsites=2,
+                      inserter_kwargs=dict(trunc=dict(cutoff=1e-5, maxdim=40)))
+    assert abs(En - ex) < 1e-5
+    ref = otree.tree_dmrg(V, E, W, psi0, nsweeps=5, maxdim=40, cutoff=1e-5)
+    assert abs(En - ref.energy) < 1e-10 * abs(ex)
+    for e in E:
+        assert psi.linkdim(*e) == ref.psi[e[0]].shape[T.adj[e[0]].index(e[1])]
+    assert abs(ns.expect_mpo(psi, H) - otree.expectation(T, W, ref.psi)) < 1e-9
+
+
+def test_tree_dmrg_region_log_parity(ctx):
+    import networksolvers_amd as ns
+    from networksolvers_amd.sweep import EigsolveProblem, eigsolve_problem, projected_operator
+    V, E = build_tree()
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.heisenberg_tree_mpo(G)
+    psi0 = neel(T, V)
+    H = ns.TreeTTNO(ctx, G, W)
+    prob = EigsolveProblem(state=ns.TreeTTN.from_arrays(ctx, G, psi0), operator=projected_operator(H))
+    kw = dict(trunc=dict(cutoff=1e-10, maxdim=[4, 8, 16]))
+    eigsolve_problem(prob, nsweeps=3, nsites=2, inserter_kwargs=kw)
+    ref = otree.tree_dmrg(V, E, W, psi0, nsweeps=3, maxdim=[4, 8, 16], cutoff=1e-10)
+    assert len(prob.region_log) == len(ref.log)
+    for (rg, e, k, te, _), (rr, er, kr, ter) in zip(prob.region_log, ref.log):
+        assert tuple(rg) == tuple(rr)
+        assert k == kr
+        assert abs(e - er) < 1e-10 * max(1.0, abs(er))
+        assert abs(te - ter) < 1e-12 + 1e-7 * ter
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_tree_tdvp_exact_at_full_rank(ctx, order):
+    import networksolvers_amd as ns
+    V, E = binary_tree(2)
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.tfim_tree_mpo(G, h=1.0)
+    psi = full_rank_random(T, 5)
+    H = ns.TreeTTNO(ctx, G, W)
+    out = ns.tdvp(H, ns.TreeTTN.from_arrays(ctx, G, psi), [0.0, 0.05, 0.1], nsites=2, tdvp_order=order,
+                  updater_kwargs=dict(solver=ns.exponentiate_solver))
+    want = sl.expm(-0.1j * otree.mpo_to_dense(T, W)) @ otree.ttn_to_dense(T, psi)
+    assert np.linalg.norm(otree.ttn_to_dense(T, out.to_arrays()) - want) < 1e-9
+
+
+def test_tree_tdvp_1site_vs_oracle(ctx):
+    """1-site tree TDVP (QR + backward bond evolution on the first edge of the
+    path to the next region): equal to the oracle; norm and energy conserved."""
+    import networksolvers_amd as ns
+    V, E = binary_tree(2)
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.tfim_tree_mpo(G, h=1.0)
+    psi = full_rank_random(T, 6)
+    H = ns.TreeTTNO(ctx, G, W)
+    ts = [0.0, 0.1, 0.3]
+    out = ns.tdvp(H, ns.TreeTTN.from_arrays(ctx, G, psi), ts, nsites=1, tdvp_order=4,
+                  updater_kwargs=dict(solver=ns.exponentiate_solver))
+    ref = otree.tree_applyexp(V, E, W, psi, [-1j * t for t in ts], nsites=1, tdvp_order=4)
+    got = otree.ttn_to_dense(T, out.to_arrays())
+    want = otree.ttn_to_dense(T, ref.psi)
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)
+    assert 1 - fid < 1e-9
+    Hd = otree.mpo_to_dense(T, W)
+    v0 = otree.ttn_to_dense(T, psi)
+    assert abs(np.vdot(got, Hd @ got).real - np.vdot(v0, Hd @ v0).real) < 1e-10
+    assert abs(np.linalg.norm(got) - 1) < 1e-12
+
+
+def test_tree_tdvp_reference_regression(ctx):
+    """test_tree_tdvp.jl:24-77 on the device (chain of 10 plus an ancilla on site
+    5, Heisenberg on the chain edges, DMRG ground state, TDVP to t = 0.1 with
+    the default Runge-Kutta solver): norms > 0.999, overlaps > 0.99, phase E t."""
+    import networksolvers_amd as ns
+    V, E = chain_plus_ancilla(10)
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.heisenberg_tree_mpo(G, edges=[(j, j + 1) for j in range(1, 10)])
+    H = ns.TreeTTNO(ctx, G, W)
+    trunc = dict(cutoff=1e-10, maxdim=100)
+    En, gs = ns.dmrg(H, ns.TreeTTN.from_arrays(ctx, G, neel(T, V)), nsweeps=5, nsites=2,
+                     inserter_kwargs=dict(trunc=trunc))
+    times = [0.0, 0.02, 0.04, 0.06, 0.08, 0.10]
+    p1 = ns.tdvp(H, gs, times, nsites=1, inserter_kwargs=dict(trunc=trunc)).to_arrays()
+    p2 = ns.tdvp(H, gs, times, nsites=2, inserter_kwargs=dict(trunc=trunc)).to_arrays()
+    g = gs.to_arrays()
+    assert np.sqrt(abs(otree.inner(T, p1, p1))) > 0.999
+    assert np.sqrt(abs(otree.inner(T, p2, p2))) > 0.999
+    z = otree.inner(T, p1, g)
+    assert abs(z) > 0.99
+    assert abs(otree.inner(T, p1, p2)) > 0.99
+    assert abs(np.arctan(z.imag / z.real) - En * 0.1) < 1e-4
+    # the same run on the oracle
+    ref = otree.tree_applyexp(V, E, W, g, [-1j * t for t in times], nsites=2, maxdim=100, cutoff=1e-10,
+                              solver="rk4")
+    fid = abs(otree.inner(T, ref.psi, p2)) ** 2 / abs(otree.inner(T, ref.psi, ref.psi) * otree.inner(T, p2, p2))
+    assert 1 - fid < 1e-9
+
+
+def test_c4_tfim_binary_tree_quench_steps(ctx):
+    """The configured tree quench (TFIM h=1 on the 63-site depth-5 binary tree,
+    all-up start, 2-site TDVP order 4, dt 0.05, maxdim 256, cutoff 1e-10,
+    exponentiate solver) for its first 4 steps against the oracle: same bond
+    dimensions, fidelity > 1 - 1e-9, and the <Z_v> profile to 1e-9."""
+    import networksolvers_amd as ns
+    V, E = binary_tree(5)
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.tfim_tree_mpo(G, h=1.0)
+    up = {}
+    for v in V:
+        k = len(T.adj[v])
+        a = np.zeros([1] * k + [2])
+        a[(0,) * k + (0,)] = 1.0
+        up[v] = a
+    H = ns.TreeTTNO(ctx, G, W)
+    times = [0.05 * k for k in range(5)]
+    out = ns.tdvp(H, ns.TreeTTN.from_arrays(ctx, G, up), times, nsites=2, tdvp_order=4,
+                  updater_kwargs=dict(solver=ns.exponentiate_solver),
+                  inserter_kwargs=dict(trunc=dict(cutoff=1e-10, maxdim=256)))
+    ref = otree.tree_applyexp(V, E, W, up, [-1j * t for t in times], nsites=2, tdvp_order=4, maxdim=256,
+                              cutoff=1e-10)
+    got = out.to_arrays()
+    for (a, b) in E:
+        assert got[a].shape[T.adj[a].index(b)] == ref.psi[a].shape[T.adj[a].index(b)]
+    fid = abs(otree.inner(T, ref.psi, got)) ** 2 / abs(otree.inner(T, ref.psi, ref.psi) * otree.inner(T, got, got))
+    assert 1 - fid < 1e-9
+    Z = np.diag([1.0, -1.0])
+    zg = ns.expect_local(out, Z)
+    zr = otree.local_expectations(T, ref.psi, Z)
+    assert max(abs(zg[v] - zr[v]) for v in V) < 1e-9
