@@ -82,6 +82,16 @@ class TTNO:
         self.tensors = [t if isinstance(t, DeviceTensor) else ctx.upload(np.ascontiguousarray(t, dtype=np.float64))
                         for t in tensors]
         self.bond_dims = [t.shape[1] for t in self.tensors[:-1]]
+        self._typed = {}
+
+    def tensor(self, v, dtype=np.float64):
+        """W of vertex v in the given element type (complex copies cached)."""
+        if np.dtype(dtype) == np.float64:
+            return self.tensors[v]
+        key = (v, np.dtype(dtype).str)
+        if key not in self._typed:
+            self._typed[key] = self.ctx.upload(self.tensors[v].numpy().astype(dtype))
+        return self._typed[key]
 
     def __len__(self):
         return len(self.tensors)

@@ -24,7 +24,7 @@ import functools
 import itertools
 from dataclasses import dataclass, field
 
-from . import kernels, tree
+from . import kernels, subspace, tree
 from .network import ProjTTN, TTN, TTNO, orthogonalize
 from .region_plans import euler_sweep, truncation_parameters
 
@@ -185,17 +185,20 @@ def sweep_solve(sweep_iter: SweepIterator, *, outputlevel=0, region_callback=def
 # -------------------------------------------------------------- region phases
 def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=None, **kws):
     """extracter.jl:3-17: orthogonalize to the region, theta = product of the region
-    tensors, (subspace expansion hook), then ``position`` builds the environments."""
-    truncation_parameters(sweep, **(trunc or {}))
+    tensors, subspace expansion (``subspace_algorithm``: None or "densitymatrix",
+    subspace/densitymatrix.jl), then ``position`` builds the environments."""
+    t = truncation_parameters(sweep, **(trunc or {}))
     region = current_region(region_iter)
-    if subspace_algorithm is not None:
+    if subspace_algorithm not in (None, "densitymatrix"):
         raise NotImplementedError(
             "Subspace expansion (subspace_expand!) not defined for requested combination of "
             f"subspace_algorithm={subspace_algorithm!r} and problem types")
     if isinstance(state(problem), tree.TreeTTN):
         psi = state(problem)
         local_state = tree.extract(psi, list(region))
-        problem.operator = operator(problem).position(psi, list(region))
+        problem, local_state = subspace.subspace_expand(problem, local_state, region,
+                                                        subspace_algorithm=subspace_algorithm, trunc=t, **kws)
+        problem.operator = operator(problem).position(problem.state, list(region))
         return problem, local_state
     psi = orthogonalize(state(problem), region)
     problem.state = psi
@@ -208,7 +211,9 @@ def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=Non
         local_state = psi[lo].copy()
     else:
         raise ValueError(f"Region of length {len(region)} not currently supported")
-    problem.operator = operator(problem).position(psi, region)
+    problem, local_state = subspace.subspace_expand(problem, local_state, region,
+                                                    subspace_algorithm=subspace_algorithm, trunc=t, **kws)
+    problem.operator = operator(problem).position(problem.state, region)
     return problem, local_state
 
 

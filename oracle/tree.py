@@ -25,7 +25,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import linalg
-from .plans import compute_adjacencies, euler_tour_edges, get_or_last, tdvp_regions
+from .plans import compute_adjacencies, euler_tour_edges, euler_tour_vertices, get_or_last, tdvp_regions
 from .tdvp import krylov_expm, runge_kutta_4
 
 
@@ -260,46 +260,152 @@ class TreeResult:
     current_time: complex = 0.0
 
 
-def tree_dmrg(vertices, edges, W, psi0, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, krylovdim=3, tol=1e-14,
-              root_vertex=None, which_decomp=None):
-    """2-site DMRG on a tree (eigsolve.jl:14-76) over the Euler-tour plan
-    (euler_plans.jl:4-13); log rows (region, eigenvalue, kept, truncerr)."""
+def compute_expansion(current_dim, basis_size, expansion_factor=1.5, max_expand=None, maxdim=None):
+    """subspace.jl:28-48."""
+    n = int(np.ceil(expansion_factor * current_dim))
+    if max_expand is not None:
+        n = min(max_expand, n)
+    n = min(basis_size - current_dim, n)
+    if maxdim is not None:
+        n = min(maxdim - current_dim, n)
+    return max(0, n)
+
+
+def densitymatrix_expand(T, W, envs, psi, prev_pos, region, local, local_labels, *, cutoff, maxdim, mindim,
+                         expansion_factor=1.5, max_expand=None, north_pass=1):
+    """subspace/densitymatrix.jl:53-122: enlarge the link between the operator's
+    previous position p and the region by the dominant eigenvectors of
+    rho = S S^dag, S = (1 - P_A) (environments of p off the region . W_p . A),
+    taken with the eigen-route truncation (cutoff, maxdim = the expansion size);
+    A <- [A | U], the next vertex and the local tensor padded by A^dag [A | U].
+    Returns (local, changed vertices)."""
+    prev = [v for v in prev_pos if v not in region]
+    if len(prev) != 1:
+        return local, []
+    p = prev[0]
+    nxt = [v for v in region if v in T.adj[p]]
+    if not nxt:
+        return local, []
+    v = nxt[0]
+    A, la = psi[p], T.labels(p)
+    e = T.key(p, v)
+    a = ("k", e)
+    ia = la.index(a)
+    basis = int(np.prod([n for i, n in enumerate(A.shape) if i != ia]))
+    nexp = compute_expansion(A.shape[ia], basis, expansion_factor, max_expand, maxdim)
+    if nexp <= 0:
+        return local, []
+    S, ls = A, list(la)
+    for n in T.adj[p]:
+        if n not in region:
+            en = T.key(n, p)
+            S, ls = ltdot(S, ls, envs.get(psi, n, p), [("b", en), ("w", en), ("k", en)])
+    S, ls = ltdot(S, ls, W[p].astype(np.result_type(W[p], A)), T.wlabels(p))
+    lab = [_bra(x) for x in la]
+    for _ in range(north_pass):
+        X, lx = ltdot(A.conj(), lab, S, ls)
+        Y, ly = ltdot(A, lab, X, lx)
+        S = S - lperm(Y, ly, ls)
+    rows = [x for x in ls if x[0] in ("b", "t")]
+    cols = [x for x in ls if x not in rows]
+    M = lperm(S, ls, rows + cols)
+    nr = int(np.prod(M.shape[:len(rows)]))
+    U, _, _ = linalg.factorize(M.reshape(nr, -1), cutoff=cutoff, maxdim=nexp, mindim=mindim, which_decomp="eigen")
+    U = U.reshape(M.shape[:len(rows)] + (U.shape[1],))
+    lu = [(("k",) + x[1:]) if x[0] == "b" else (("s",) + x[1:]) for x in rows] + [("x", e)]
+    laa = [_bra(x) if x == a else x for x in la]
+    for _ in range(north_pass):
+        X, lx = ltdot(A.conj(), laa, U, lu)
+        Y, ly = ltdot(A, laa, X, lx)
+        U = U - lperm(Y, ly, lu)
+    X, _ = ltdot(U.conj(), lu, A, la)
+    if np.linalg.norm(X) > 1e-10:
+        return local, []
+    lo = [x for x in la if x != a]
+    Ap = lperm(A, la, lo + [a])
+    Up = lperm(U, lu, lo + [("x", e)])
+    Ax = np.concatenate([Ap, Up], axis=-1)                  # directsum(A => a, U => new)
+    expander = np.tensordot(Ax.conj(), Ap, axes=(list(range(len(lo))), list(range(len(lo)))))   # [ax, a]
+    psi[p] = lperm(Ax, lo + [a], la)
+    Cn, lc = ltdot(expander, [("xa", e), a], psi[v], T.labels(v))
+    psi[v] = lperm(Cn, [a if x == ("xa", e) else x for x in lc], T.labels(v))
+    Ln, ll = ltdot(expander, [("xa", e), a], local, local_labels)
+    return lperm(Ln, [a if x == ("xa", e) else x for x in ll], local_labels), [p, v]
+
+
+def tree_dmrg(vertices, edges, W, psi0, *, nsweeps, nsites=2, maxdim=None, cutoff=0.0, mindim=1, krylovdim=3,
+              tol=1e-14, root_vertex=None, which_decomp=None, subspace_algorithm=None, expansion_factor=1.5,
+              max_expand=None, north_pass=1, expand_trunc=None):
+    """DMRG on a tree (eigsolve.jl:14-76) over the Euler-tour plan (euler_plans.jl:4-13;
+    edges for nsites=2, vertices for nsites=1), with the densitymatrix subspace
+    expansion in the extracter when asked (extracter.jl:10-12, using the
+    operator's previous position); log rows (region, eigenvalue, kept, truncerr).
+    ``expand_trunc`` (dict cutoff/maxdim/mindim) is the extracter's truncation;
+    by default the inserter's."""
     T = Tree(vertices, edges)
-    psi = {v: np.array(a, dtype=np.float64) for v, a in psi0.items()}
+    cplx = any(np.iscomplexobj(a) for a in psi0.values())
+    psi = {v: np.array(a, dtype=np.complex128 if cplx else np.float64) for v, a in psi0.items()}
     envs = TreeEnvs(T, W)
     root = default_root(T.vertices, T.edges) if root_vertex is None else root_vertex
     ortho = None
+    pos = list(T.vertices)
     log, E = [], None
+    if nsites == 2:
+        plan = [list(r) for r in euler_tour_edges(T.vertices, T.edges, root)]
+    else:
+        plan = [[x] for x in euler_tour_vertices(T.vertices, T.edges, root)]
     for sweep in range(1, nsweeps + 1):
         cut = get_or_last(cutoff, sweep)
         mxd = get_or_last(maxdim, sweep) if maxdim is not None else None
         mnd = get_or_last(mindim, sweep)
-        for (u, v) in euler_tour_edges(T.vertices, T.edges, root):
-            region = [u, v]
+        et = expand_trunc or {}
+        ecut = get_or_last(et.get("cutoff", cutoff), sweep)
+        emx = et.get("maxdim", maxdim)
+        emx = get_or_last(emx, sweep) if emx is not None else None
+        emn = get_or_last(et.get("mindim", mindim), sweep)
+        for region in plan:
             before = {x: psi[x] for x in T.vertices}
             ortho = orthogonalize(T, psi, ortho, region)
             for x in T.vertices:
                 if psi[x] is not before[x]:
                     envs.touch(x)
-            th, _ = ltdot(psi[u], T.labels(u), psi[v], T.labels(v))
+            if len(region) == 2:
+                u, v = region
+                th, _ = ltdot(psi[u], T.labels(u), psi[v], T.labels(v))
+            else:
+                th = psi[region[0]]
+            if subspace_algorithm == "densitymatrix":
+                th, changed = densitymatrix_expand(T, W, envs, psi, pos, region, th, region_labels(T, region),
+                                                   cutoff=ecut, maxdim=emx, mindim=emn,
+                                                   expansion_factor=expansion_factor, max_expand=max_expand,
+                                                   north_pass=north_pass)
+                for x in changed:
+                    envs.touch(x)
+            pos = list(region)
             envs_in = {(n, r): envs.get(psi, n, r) for r in region for n in T.adj[r] if n not in region}
             shape = th.shape
             E, x, _ = linalg.lanczos_eigsolve(
                 lambda y: heff_apply(T, W, envs_in, region, y.reshape(shape)).reshape(-1), th.reshape(-1),
                 krylovdim=krylovdim, tol=tol)
             x = x.reshape(shape)
-            e = T.key(u, v)
-            lu = [y for y in T.labels(u) if y != ("k", e)]
-            lv = [y for y in T.labels(v) if y != ("k", e)]
-            nu = int(np.prod([shape[i] for i in range(len(lu))]))
-            U, C, info = linalg.factorize(x.reshape(nu, -1), cutoff=cut, maxdim=mxd, mindim=mnd,
-                                          which_decomp=which_decomp)
-            psi[u] = lperm(U.reshape(shape[:len(lu)] + (U.shape[1],)), lu + [("k", e)], T.labels(u))
-            psi[v] = lperm(C.reshape((C.shape[0],) + shape[len(lu):]), [("k", e)] + lv, T.labels(v))
-            envs.touch(u)
+            if len(region) == 2:
+                u, v = region
+                e = T.key(u, v)
+                lu = [y for y in T.labels(u) if y != ("k", e)]
+                lv = [y for y in T.labels(v) if y != ("k", e)]
+                nu = int(np.prod([shape[i] for i in range(len(lu))]))
+                U, C, info = linalg.factorize(x.reshape(nu, -1), cutoff=cut, maxdim=mxd, mindim=mnd,
+                                              which_decomp=which_decomp)
+                psi[u] = lperm(U.reshape(shape[:len(lu)] + (U.shape[1],)), lu + [("k", e)], T.labels(u))
+                psi[v] = lperm(C.reshape((C.shape[0],) + shape[len(lu):]), [("k", e)] + lv, T.labels(v))
+                envs.touch(u)
+                log.append((tuple(region), E, info.kept, info.truncerr))
+            else:
+                v = region[0]
+                psi[v] = x
+                log.append((tuple(region), E, None, None))
             envs.touch(v)
             ortho = [v]
-            log.append((tuple(region), E, info.kept, info.truncerr))
     return TreeResult(E, psi, log)
 
 

@@ -191,3 +191,50 @@ def test_tree_post_order_plan():
     assert order == [3, 4, 1, 0, 5, 2, 6]
     assert [tuple(e) for e in plans.post_order_dfs_edges(V, E, 6)] == [(3, 1), (4, 1), (1, 0), (0, 2), (5, 2),
                                                                       (2, 6)]
+
+
+def test_compute_expansion_closed_forms():
+    """subspace.jl:28-48: ceil(factor * dim), then the max_expand, basis and maxdim caps, floored at 0."""
+    assert otree.compute_expansion(10, 1000) == 15
+    assert otree.compute_expansion(10, 1000, expansion_factor=1.1) == 11
+    assert otree.compute_expansion(10, 1000, max_expand=4) == 4
+    assert otree.compute_expansion(10, 18) == 8
+    assert otree.compute_expansion(10, 1000, maxdim=12) == 2
+    assert otree.compute_expansion(10, 1000, maxdim=8) == 0
+    assert otree.compute_expansion(6, 6) == 0
+
+
+def test_tree_dmrg_1site_densitymatrix_reference_case():
+    """test_tree_dmrg.jl:78-90: 1-site DMRG with densitymatrix subspace expansion
+    (trunc cutoff 1e-5, maxdim 40, default expansion factor 1.5) reaches the ED
+    energy (reference bar 1e-5)."""
+    V, E = build_tree()
+    T = otree.Tree(V, E)
+    W = _mpo(V, E, "heis")
+    ex = _ed("tree", "build_tree_3_3_heisenberg_sz0")
+    res = otree.tree_dmrg(V, E, W, neel(T, V), nsweeps=5, nsites=1, maxdim=40, cutoff=1e-5,
+                          subspace_algorithm="densitymatrix")
+    assert abs(res.energy - ex) < 1e-5
+    assert abs(otree.expectation(T, W, res.psi) - res.energy) < 1e-10
+
+
+def test_path_dmrg_densitymatrix_reference_example():
+    """examples/dmrg.jl:10-43 (S=1 chain N=10, 2-site with densitymatrix expansion,
+    expansion factor 1.1, cutoff 1e-12, maxdim [10,40,80,160], 5 sweeps) against the
+    reference's exact energy -12.8945601 and the ED fixture, on the path graph."""
+    from oracle import models as omodels
+    N = 10
+    V, E = list(range(N)), [(i, i + 1) for i in range(N - 1)]
+    Wc = omodels.heisenberg_mpo(N, d=3)
+    W = {0: Wc[0][0], N - 1: Wc[N - 1][:, 0]}
+    W.update({i: Wc[i] for i in range(1, N - 1)})
+    psi = {}
+    for i in range(N):
+        k = 1 if i in (0, N - 1) else 2
+        a = np.zeros([1] * k + [3])
+        a[(0,) * k + (0 if (i + 1) % 2 == 0 else 2,)] = 1.0
+        psi[i] = a
+    res = otree.tree_dmrg(V, E, W, psi, nsweeps=5, nsites=2, maxdim=[10, 40, 80, 160], cutoff=1e-12,
+                          subspace_algorithm="densitymatrix", expansion_factor=1.1)
+    assert abs(res.energy - (-12.8945601)) < 1e-7
+    assert abs(res.energy - _ed("heisenberg_s1_chain", "10")) < 1e-9
