@@ -43,6 +43,9 @@ def parse():
     p.add_argument("--no-fp64-plain", action="store_true", help="skip the FP64-plain comparison sweep")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU baseline sample")
+    p.add_argument("--serving-requests", type=int, default=40,
+                   help="Poisson-stream requests of the serving leg (0 skips it)")
+    p.add_argument("--serving-slots", type=int, default=8)
     return p.parse_args()
 
 
@@ -158,6 +161,48 @@ def cpu_baseline(args, budget_s, mps_host):
                       f"{1e3 * t_region:.1f} ms each, numpy/OpenBLAS FP64 restatement; first region excluded"}
 
 
+def serving_leg(args):
+    """The mixed-size serving stream (XXZ chains N ~ U{16..128}, Delta ~ U[0,2],
+    maxdim ~ U{64,128,256}, 4 sweeps, cutoff 1e-10, FP32 H_eff form): saturated
+    throughput from a burst queued at t=0, then a Poisson stream at 0.8x that
+    rate for p50/p99 latency; a few requests are re-solved in FP64 for the
+    energy error of the form."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import serving
+    srv = serving.Server(slots=args.serving_slots)
+    try:
+        burst = serving.make_stream(max(2 * args.serving_slots, 8), rate=0.0, seed=args.seed + 1)
+        res_b, wall_b = srv.run(burst, realtime=False)
+        sat = len([r for r in res_b if r.error is None]) / wall_b
+        rate = 0.8 * sat
+        stream = serving.make_stream(args.serving_requests, rate=rate, seed=args.seed + 2)
+        print(f"[bench] serving: saturated {sat:.2f} solves/s; stream of {len(stream)} at {rate:.2f}/s",
+              file=sys.stderr, flush=True)
+        res, wall = srv.run(stream, realtime=True)
+    finally:
+        srv.close()
+    ok = [r for r in res if r.error is None]
+    lat = serving.latency_stats(res)
+    # FP64 re-solve of a few requests: energy error of the FP32 form
+    ctx = ns.Context(0)
+    errs = []
+    try:
+        for r in ok[:4]:
+            req = stream[r.rid]
+            req64 = serving.SolveRequest(**{**req.__dict__, "precision": "fp64"})
+            e64, _, _ = serving.run_solve(ctx, req64)
+            errs.append(abs(r.energy - e64) / abs(e64))
+    finally:
+        ctx.close()
+    return {"workload": "XXZ chains N~U{16..128}, Delta~U[0,2], maxdim~U{64,128,256}, 4 sweeps, cutoff 1e-10, "
+                        "fp32 H_eff + f64 Krylov/factorize",
+            "requests": len(stream), "completed": len(ok), "slots": args.serving_slots,
+            "saturated_solves_per_s": round(sat, 3), "arrival_rate_per_s": round(rate, 3),
+            "solves_per_s": round(len(ok) / wall, 3), "p50_latency_s": lat["p50_s"], "p99_latency_s": lat["p99_s"],
+            "mean_latency_s": lat["mean_s"], "max_rel_dE_vs_fp64": max(errs) if errs else None,
+            "peak_concurrency": srv.peak_concurrency}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -204,6 +249,8 @@ def main():
                               "speedup_of_form": round(value * r["t_fp64_sweep"], 3),
                               "E_fp64_after_sweep": r["E_fp64"],
                               "rel_dE": abs(r["E_mixed"] - r["E_fp64"]) / abs(r["E_fp64"])}
+    if args.serving_requests > 0:
+        line["serving"] = serving_leg(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args, args.cpu_seconds, r["mps_host"])
     print(json.dumps(line), flush=True)
