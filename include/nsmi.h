@@ -222,6 +222,40 @@ int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* o
                       const nsmi_net_step* steps, int x_rank, const int64_t* x_dims, const int* out_perm,
                       nsmi_heff** out);
 
+/* ------------------------------------------ quantum-number (block-sparse) */
+/* A block program: the block operations of one contraction of U(1) block-sparse
+ * tensors (ITensors QN ITensors as used by the reference's QN sweeps, README.md
+ * QN timings; operator_map.jl:3-10 "per compatible block pair", extracter.jl:7,14,
+ * inserter.jl:23).  Buffers: 0 = x, 1 = y, 2.. = operands (FP64 tensors), then
+ * n_scratch scratch buffers of scratch_elems[i] doubles.  steps: n_steps rows of
+ * (kind, transA, transB, first item, item count, 0); kind 0 = grouped GEMM over
+ * gemm_items rows (c_buf, c_off, ldc, M, N, first segment, segment count) whose
+ * segments are segs rows (a_buf, a_off, lda, b_buf, b_off, ldb, K) scaled by
+ * seg_alpha -- each output block is the sum of its segments, written once; kind 1
+ * = linear combinations over lin_items rows (out_buf, out_off, n, first term,
+ * term count) of terms rows (src_buf, src_off) times term_coeff; kind 2 = copies
+ * over copy_items rows (src_buf, src_off, src_ld, dst_buf, dst_off, dst_ld, rows,
+ * cols).  Every block is checked against its buffer here.  With x_elems ==
+ * y_elems the handle is an effective Hamiltonian for nsmi_heff_apply/_rayleigh,
+ * nsmi_lanczos_eig and nsmi_krylov_expm; any program runs with nsmi_bsop_run. */
+int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_scratch,
+                     const int64_t* scratch_elems, int64_t x_elems, int64_t y_elems, int n_steps,
+                     const int64_t* steps, const int64_t* gemm_items, const int64_t* segs, const double* seg_alpha,
+                     const int64_t* lin_items, const int64_t* terms, const double* term_coeff,
+                     const int64_t* copy_items, nsmi_heff** out);
+int nsmi_bsop_run(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y);
+
+/* Truncated factorisation of a block-diagonal matrix (ITensors factorize of a QN
+ * ITensor, inserter.jl:23): sector q is the rows[q] x cols[q] row-major matrix at
+ * offsets[q] of Mall.  Every sector is decomposed as nsmi_factorize does; the
+ * spectra are merged and truncated together (truncate! on the sorted union) and
+ * each sector keeps its entries above the cut value (NDTensors block-sparse
+ * svd/eigen).  kept[q] receives the kept count; Uall / Call hold U_q (rows x k,
+ * or k x cols when rows_trailing) and C_q concatenated in sector order. */
+int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const int64_t* offsets, const int64_t* rows,
+                           const int64_t* cols, int rows_trailing, const nsmi_trunc* trunc, nsmi_tensor** Uall,
+                           nsmi_tensor** Call, int64_t* kept, nsmi_trunc_info* info);
+
 /* ------------------------------------------------------------- profiling */
 int nsmi_profile_enable(nsmi_ctx* ctx, int on);
 int nsmi_profile_reset(nsmi_ctx* ctx);

@@ -21,6 +21,7 @@ from .sweep import (EigsolveProblem, dmrg, eigsolve, eigsolve_solver, extracter,
 from .applyexp import (ApplyExpProblem, applyexp, current_time, exponentiate_solver,  # noqa: E402
                        process_real_times, runge_kutta_solver, tdvp)
 from .kernels import NetMap, permute, tensordot  # noqa: E402
+from .qn import QNIndex, QNMPO, QNMPS, QNProjTTN, QNTensor, qn_product_mps, spin_site_index  # noqa: E402
 from .tree import (TreeGraph, TreeProjTTN, TreeTTN, TreeTTNO, heisenberg_tree_mpo, product_ttn,  # noqa: E402
                    random_ttn, tfim_tree_mpo, tree_mpo)
 
@@ -33,5 +34,5 @@ __all__ = [
     "sweep_solve", "updater", "ApplyExpProblem", "applyexp", "current_time", "exponentiate_solver",
     "process_real_times", "runge_kutta_solver", "tdvp", "NetMap", "permute", "tensordot", "TreeGraph",
     "TreeProjTTN", "TreeTTN", "TreeTTNO", "heisenberg_tree_mpo", "product_ttn", "random_ttn", "tfim_tree_mpo",
-    "tree_mpo",
+    "tree_mpo", "QNIndex", "QNMPO", "QNMPS", "QNProjTTN", "QNTensor", "qn_product_mps", "spin_site_index",
 ]

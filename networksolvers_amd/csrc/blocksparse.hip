@@ -1,0 +1,457 @@
+// Block-sparse (U(1) quantum-number) contractions as device "block programs".
+//
+// A QN tensor is a set of dense blocks (one per allowed charge combination) in one
+// flat FP64 buffer.  Every contraction the sweep needs on such tensors (theta,
+// environment updates, the H_eff apply, sector assembly for the factorisation) is
+// a fixed list of block operations, built once by the host from the block tables:
+//   GEMM step  : output block C (M x N) = sum over its segments alpha_i op(A_i) op(B_i)
+//                (one grouped launch; a workgroup owns one output tile and walks the
+//                tile's segments, so blocks that receive several contributions are
+//                summed in registers, with no atomics and no extra pass)
+//   LINCOMB    : output block = sum_j c_j * source block_j (same shapes; the MPO mixing)
+//   COPY       : 2-D strided block copies (gather into / scatter out of sector matrices)
+// Buffers are numbered: 0 = x (input), 1 = y (output), 2.. = operands, then scratch.
+// Every offset is checked against its buffer's size when the program is created (x
+// and y when it runs), so a kernel never sees an out-of-range block.
+#include <algorithm>
+#include <numeric>
+
+#include "internal.h"
+
+namespace nsmi {
+namespace {
+
+constexpr int kMaxBufs = 32;
+
+struct BufTable {
+  double* p[kMaxBufs];
+};
+
+struct GItem {  // one output block
+  int c_buf;
+  int seg_first, seg_count;
+  int pad;
+  int64_t c_off, ldc, M, N;
+};
+struct GSeg {
+  int a_buf, b_buf;
+  int64_t a_off, lda, b_off, ldb, K;
+  double alpha;
+};
+struct GTile {
+  int item, tm, tn, pad;
+};
+struct LItem {
+  int out_buf, term_first, term_count, pad;
+  int64_t out_off, n;
+};
+struct LTerm {
+  int src_buf, pad;
+  int64_t src_off;
+  double coeff;
+};
+struct CItem {
+  int src_buf, dst_buf;
+  int64_t src_off, src_ld, dst_off, dst_ld, rows, cols;
+};
+struct Chunk {  // 256 consecutive elements of one lincomb / copy item
+  int item, pad;
+  int64_t start;
+};
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_bs_gemm(BufTable bt, const GItem* __restrict__ items,
+                                                 const GSeg* __restrict__ segs, const GTile* __restrict__ tiles) {
+  constexpr int BK = 16, PAD = 4;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
+  __shared__ double sA[BK][BM + PAD];
+  __shared__ double sB[BK][BN + PAD];
+  const GTile t = tiles[blockIdx.x];
+  const GItem it = items[t.item];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t m0 = int64_t(t.tm) * BM, n0 = int64_t(t.tn) * BN;
+  const int kl = lane >> 4, ml = lane & 15;
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+  for (int s = 0; s < it.seg_count; ++s) {
+    const GSeg sg = segs[it.seg_first + s];
+    const double* __restrict__ A = bt.p[sg.a_buf] + sg.a_off;
+    const double* __restrict__ B = bt.p[sg.b_buf] + sg.b_off;
+    for (int64_t k0 = 0; k0 < sg.K; k0 += BK) {
+#pragma unroll
+      for (int j = 0; j < EA; ++j) {
+        const int e = tid * EA + j;
+        const int mm = TA ? (e % BM) : (e / BK);
+        const int kk = TA ? (e / BM) : (e % BK);
+        const int64_t gm = m0 + mm, gk = k0 + kk;
+        double v = 0.0;
+        if (gm < it.M && gk < sg.K) v = sg.alpha * (TA ? A[gk * sg.lda + gm] : A[gm * sg.lda + gk]);
+        sA[kk][mm] = v;
+      }
+#pragma unroll
+      for (int j = 0; j < EB; ++j) {
+        const int e = tid * EB + j;
+        const int nn = TB ? (e / BK) : (e % BN);
+        const int kk = TB ? (e % BK) : (e / BN);
+        const int64_t gn = n0 + nn, gk = k0 + kk;
+        double v = 0.0;
+        if (gn < it.N && gk < sg.K) v = TB ? B[gn * sg.ldb + gk] : B[gk * sg.ldb + gn];
+        sB[kk][nn] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        double af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = sA[ks * 4 + kl][wm * WTM + i * 16 + ml];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = sB[ks * 4 + kl][wn * WTN + j * 16 + ml];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+  double* __restrict__ C = bt.p[it.c_buf] + it.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + wm * WTM + i * 16 + kl + 4 * r;  // f64 16x16x4 output rows interleave
+        const int64_t gn = n0 + wn * WTN + j * 16 + ml;
+        if (gm < it.M && gn < it.N) C[gm * it.ldc + gn] = acc[i][j][r];
+      }
+}
+
+__global__ __launch_bounds__(256) void k_bs_lincomb(BufTable bt, const LItem* __restrict__ items,
+                                                    const LTerm* __restrict__ terms, const Chunk* __restrict__ chunks) {
+  const Chunk c = chunks[blockIdx.x];
+  const LItem it = items[c.item];
+  const int64_t i = c.start + threadIdx.x;
+  if (i >= it.n) return;
+  double acc = 0.0;
+  for (int t = 0; t < it.term_count; ++t) {
+    const LTerm tm = terms[it.term_first + t];
+    acc += tm.coeff * bt.p[tm.src_buf][tm.src_off + i];
+  }
+  bt.p[it.out_buf][it.out_off + i] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_bs_copy(BufTable bt, const CItem* __restrict__ items,
+                                                 const Chunk* __restrict__ chunks) {
+  const Chunk c = chunks[blockIdx.x];
+  const CItem it = items[c.item];
+  const int64_t i = c.start + threadIdx.x;
+  if (i >= it.rows * it.cols) return;
+  const int64_t r = i / it.cols, q = i % it.cols;
+  bt.p[it.dst_buf][it.dst_off + r * it.dst_ld + q] = bt.p[it.src_buf][it.src_off + r * it.src_ld + q];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ program
+struct BsStep {
+  int kind;  // 0 GEMM, 1 LINCOMB, 2 COPY
+  bool ta = false, tb = false;
+  int tile = 32;
+  // device arrays (pool blocks) and counts
+  void* items = nullptr;
+  void* aux = nullptr;    // segments (GEMM) / terms (LINCOMB)
+  void* sched = nullptr;  // tiles (GEMM) / chunks (LINCOMB, COPY)
+  int64_t nsched = 0;
+  double flops = 0.0;
+};
+
+struct BsProgram {
+  int n_operands = 0;
+  std::vector<const nsmi_tensor*> operands;
+  std::vector<void*> scratch;
+  std::vector<int64_t> sizes;  // elements per buffer id
+  std::vector<BsStep> steps;
+  int64_t x_elems = 0, y_elems = 0;
+  double flops = 0.0;
+};
+
+void bs_release(nsmi_ctx* ctx, BsProgram* p) {
+  if (!p) return;
+  for (auto& s : p->steps) {
+    scratch_release(ctx, s.items);
+    scratch_release(ctx, s.aux);
+    scratch_release(ctx, s.sched);
+  }
+  for (void* s : p->scratch) scratch_release(ctx, s);
+  delete p;
+}
+
+namespace {
+
+template <typename T>
+void* upload_vec(nsmi_ctx* ctx, const std::vector<T>& v) {
+  if (v.empty()) return nullptr;
+  void* d = scratch(ctx, sizeof(T) * v.size());
+  NSMI_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+  return d;
+}
+
+void check_range(const BsProgram* p, int buf, int64_t off, int64_t rows, int64_t ld, int64_t cols,
+                 const char* what) {
+  require(buf >= 0 && buf < int(p->sizes.size()), std::string("bsop: bad buffer id in ") + what);
+  require(off >= 0 && rows >= 0 && cols >= 0 && ld >= 0, std::string("bsop: negative extent in ") + what);
+  if (rows == 0 || cols == 0) return;
+  require(cols <= ld || rows == 1, std::string("bsop: leading dimension below the row length in ") + what);
+  const int64_t last = off + (rows - 1) * ld + cols;
+  require(last <= p->sizes[buf], std::string("bsop: block outside its buffer in ") + what);
+}
+
+template <int BM, int BN>
+void launch_gemm(nsmi_ctx* ctx, const BufTable& bt, const BsStep& s) {
+  const dim3 grid(unsigned(s.nsched)), block(256);
+  const GItem* it = static_cast<const GItem*>(s.items);
+  const GSeg* sg = static_cast<const GSeg*>(s.aux);
+  const GTile* tl = static_cast<const GTile*>(s.sched);
+  if (!s.ta && !s.tb)
+    hipLaunchKernelGGL((k_bs_gemm<BM, BN, false, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+  else if (!s.ta && s.tb)
+    hipLaunchKernelGGL((k_bs_gemm<BM, BN, false, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+  else if (s.ta && !s.tb)
+    hipLaunchKernelGGL((k_bs_gemm<BM, BN, true, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+  else
+    hipLaunchKernelGGL((k_bs_gemm<BM, BN, true, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+  NSMI_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void bs_run(nsmi_ctx* ctx, BsProgram* p, const double* x, double* y) {
+  BufTable bt{};
+  bt.p[0] = const_cast<double*>(x);
+  bt.p[1] = y;
+  for (int i = 0; i < p->n_operands; ++i) bt.p[2 + i] = static_cast<double*>(p->operands[i]->data);
+  for (size_t i = 0; i < p->scratch.size(); ++i) bt.p[2 + p->n_operands + i] = static_cast<double*>(p->scratch[i]);
+  ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, p->flops, 0.0);
+  for (const BsStep& s : p->steps) {
+    if (s.nsched == 0) continue;
+    if (s.kind == 0) {
+      if (s.tile == 64)
+        launch_gemm<64, 64>(ctx, bt, s);
+      else
+        launch_gemm<32, 32>(ctx, bt, s);
+    } else if (s.kind == 1) {
+      hipLaunchKernelGGL(k_bs_lincomb, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
+                         static_cast<const LItem*>(s.items), static_cast<const LTerm*>(s.aux),
+                         static_cast<const Chunk*>(s.sched));
+      NSMI_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(k_bs_copy, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
+                         static_cast<const CItem*>(s.items), static_cast<const Chunk*>(s.sched));
+      NSMI_LAUNCH_CHECK();
+    }
+  }
+}
+
+int64_t bs_x_elems(const BsProgram* p) { return p->x_elems; }
+
+}  // namespace nsmi
+
+using namespace nsmi;
+
+#define API_BEGIN try {
+#define API_END(ctxp)                                             \
+  }                                                               \
+  catch (const nsmi::Error& e) {                                  \
+    if (ctxp) (ctxp)->err = e.what();                             \
+    return e.code;                                                \
+  }                                                               \
+  catch (const std::exception& e) {                               \
+    if (ctxp) (ctxp)->err = std::string("internal: ") + e.what(); \
+    return NSMI_ERR_INVALID;                                      \
+  }                                                               \
+  return NSMI_OK;
+
+extern "C" {
+
+int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_scratch,
+                     const int64_t* scratch_elems, int64_t x_elems, int64_t y_elems, int n_steps,
+                     const int64_t* steps, const int64_t* gemm_items, const int64_t* segs, const double* seg_alpha,
+                     const int64_t* lin_items, const int64_t* terms, const double* term_coeff,
+                     const int64_t* copy_items, nsmi_heff** out) {
+  if (!ctx || !out) return NSMI_ERR_INVALID;
+  BsProgram* p = nullptr;
+  API_BEGIN
+  *out = nullptr;
+  require(n_operands >= 0 && n_scratch >= 0 && 2 + n_operands + n_scratch <= kMaxBufs,
+          "bsop: at most 30 operand and scratch buffers");
+  require(x_elems >= 0 && y_elems >= 0 && n_steps >= 0 && (n_steps == 0 || steps), "bsop: bad program header");
+  p = new BsProgram();
+  p->n_operands = n_operands;
+  p->x_elems = x_elems;
+  p->y_elems = y_elems;
+  p->sizes.push_back(x_elems);
+  p->sizes.push_back(y_elems);
+  for (int i = 0; i < n_operands; ++i) {
+    require(operands[i] && operands[i]->dtype == NSMI_F64, "bsop: operands must be FP64 tensors");
+    p->operands.push_back(operands[i]);
+    p->sizes.push_back(operands[i]->numel);
+  }
+  for (int i = 0; i < n_scratch; ++i) {
+    require(scratch_elems[i] >= 0, "bsop: negative scratch size");
+    p->scratch.push_back(scratch(ctx, sizeof(double) * size_t(std::max<int64_t>(scratch_elems[i], 1))));
+    p->sizes.push_back(scratch_elems[i]);
+  }
+  for (int si = 0; si < n_steps; ++si) {
+    const int64_t* st = steps + 6 * si;  // kind, transA, transB, first, count, aux
+    BsStep s;
+    s.kind = int(st[0]);
+    s.ta = st[1] != 0;
+    s.tb = st[2] != 0;
+    const int64_t first = st[3], count = st[4];
+    require(first >= 0 && count >= 0, "bsop: bad item range");
+    if (s.kind == 0) {
+      std::vector<GItem> its;
+      std::vector<GSeg> sg;
+      std::vector<GTile> tl;
+      int64_t big = 0, all = 0;
+      for (int64_t k = 0; k < count; ++k) {
+        const int64_t* g = gemm_items + 7 * (first + k);  // c_buf, c_off, ldc, M, N, seg_first, seg_count
+        GItem it{};
+        it.c_buf = int(g[0]);
+        it.c_off = g[1];
+        it.ldc = g[2];
+        it.M = g[3];
+        it.N = g[4];
+        check_range(p, it.c_buf, it.c_off, it.M, it.ldc, it.N, "a GEMM output block");
+        it.seg_first = int(sg.size());
+        it.seg_count = int(g[6]);
+        for (int64_t j = 0; j < g[6]; ++j) {
+          const int64_t* q = segs + 7 * (g[5] + j);  // a_buf, a_off, lda, b_buf, b_off, ldb, K
+          GSeg x{};
+          x.a_buf = int(q[0]);
+          x.a_off = q[1];
+          x.lda = q[2];
+          x.b_buf = int(q[3]);
+          x.b_off = q[4];
+          x.ldb = q[5];
+          x.K = q[6];
+          x.alpha = seg_alpha[g[5] + j];
+          // op(A) is M x K: stored K x M when transposed
+          if (s.ta)
+            check_range(p, x.a_buf, x.a_off, x.K, x.lda, it.M, "a GEMM A operand");
+          else
+            check_range(p, x.a_buf, x.a_off, it.M, x.lda, x.K, "a GEMM A operand");
+          if (s.tb)
+            check_range(p, x.b_buf, x.b_off, it.N, x.ldb, x.K, "a GEMM B operand");
+          else
+            check_range(p, x.b_buf, x.b_off, x.K, x.ldb, it.N, "a GEMM B operand");
+          s.flops += 2.0 * double(it.M) * double(it.N) * double(x.K);
+          sg.push_back(x);
+        }
+        its.push_back(it);
+        all += it.M * it.N;
+        if (it.M >= 48 && it.N >= 48) big += it.M * it.N;
+      }
+      s.tile = (2 * big >= all && all > 0) ? 64 : 32;
+      const int T = s.tile;
+      for (size_t k = 0; k < its.size(); ++k) {
+        const int64_t tm = (its[k].M + T - 1) / T, tn = (its[k].N + T - 1) / T;
+        for (int64_t a = 0; a < tm; ++a)
+          for (int64_t b = 0; b < tn; ++b) tl.push_back(GTile{int(k), int(a), int(b), 0});
+      }
+      require(tl.size() < (size_t(1) << 31), "bsop: too many tiles");
+      s.items = upload_vec(ctx, its);
+      s.aux = upload_vec(ctx, sg);
+      s.sched = upload_vec(ctx, tl);
+      s.nsched = int64_t(tl.size());
+    } else if (s.kind == 1) {
+      std::vector<LItem> its;
+      std::vector<LTerm> tm;
+      std::vector<Chunk> ch;
+      for (int64_t k = 0; k < count; ++k) {
+        const int64_t* l = lin_items + 5 * (first + k);  // out_buf, out_off, n, term_first, term_count
+        LItem it{};
+        it.out_buf = int(l[0]);
+        it.out_off = l[1];
+        it.n = l[2];
+        check_range(p, it.out_buf, it.out_off, 1, it.n, it.n, "a LINCOMB output block");
+        it.term_first = int(tm.size());
+        it.term_count = int(l[4]);
+        for (int64_t j = 0; j < l[4]; ++j) {
+          const int64_t* q = terms + 2 * (l[3] + j);
+          LTerm t{};
+          t.src_buf = int(q[0]);
+          t.src_off = q[1];
+          t.coeff = term_coeff[l[3] + j];
+          check_range(p, t.src_buf, t.src_off, 1, it.n, it.n, "a LINCOMB source block");
+          tm.push_back(t);
+        }
+        for (int64_t c0 = 0; c0 < it.n; c0 += 256) ch.push_back(Chunk{int(k), 0, c0});
+        its.push_back(it);
+      }
+      s.items = upload_vec(ctx, its);
+      s.aux = upload_vec(ctx, tm);
+      s.sched = upload_vec(ctx, ch);
+      s.nsched = int64_t(ch.size());
+    } else if (s.kind == 2) {
+      std::vector<CItem> its;
+      std::vector<Chunk> ch;
+      for (int64_t k = 0; k < count; ++k) {
+        const int64_t* c = copy_items + 8 * (first + k);
+        CItem it{int(c[0]), int(c[3]), c[1], c[2], c[4], c[5], c[6], c[7]};
+        check_range(p, it.src_buf, it.src_off, it.rows, it.src_ld, it.cols, "a COPY source block");
+        check_range(p, it.dst_buf, it.dst_off, it.rows, it.dst_ld, it.cols, "a COPY destination block");
+        for (int64_t c0 = 0; c0 < it.rows * it.cols; c0 += 256) ch.push_back(Chunk{int(k), 0, c0});
+        its.push_back(it);
+      }
+      s.items = upload_vec(ctx, its);
+      s.sched = upload_vec(ctx, ch);
+      s.nsched = int64_t(ch.size());
+    } else {
+      fail(NSMI_ERR_INVALID, "bsop: unknown step kind");
+    }
+    require(s.nsched < (int64_t(1) << 31), "bsop: step too large for one launch");
+    p->flops += s.flops;
+    p->steps.push_back(s);
+  }
+  *out = heff_wrap_bs(p, x_elems == y_elems ? x_elems : -1);
+  p = nullptr;
+  }
+  catch (const nsmi::Error& e) {
+    bs_release(ctx, p);
+    ctx->err = e.what();
+    return e.code;
+  }
+  catch (const std::exception& e) {
+    bs_release(ctx, p);
+    ctx->err = std::string("internal: ") + e.what();
+    return NSMI_ERR_INVALID;
+  }
+  return NSMI_OK;
+}
+
+int nsmi_bsop_run(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y) {
+  if (!ctx || !op) return NSMI_ERR_INVALID;
+  API_BEGIN
+  BsProgram* p = heff_bs_program(op);
+  require(p != nullptr, "bsop_run: not a block program");
+  require(!x || x->dtype == NSMI_F64, "bsop_run: x must be FP64");
+  require(!y || y->dtype == NSMI_F64, "bsop_run: y must be FP64");
+  require((x ? x->numel : 0) == p->x_elems || (!x && p->x_elems == 0), "bsop_run: x size does not match the program");
+  require((y ? y->numel : 0) == p->y_elems || (!y && p->y_elems == 0), "bsop_run: y size does not match the program");
+  require(!x || !y || x->data != y->data, "bsop_run: x and y must not alias");
+  bs_run(ctx, p, x ? static_cast<const double*>(x->data) : nullptr, y ? static_cast<double*>(y->data) : nullptr);
+  API_END(ctx)
+}
+
+}  // extern "C"

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <numeric>
 
 #include "internal.h"
@@ -211,6 +212,166 @@ void check_info(nsmi_ctx* ctx, const char* what) {
 }  // namespace
 }  // namespace nsmi
 
+namespace nsmi {
+namespace {
+
+// One real matrix M (m x nc) split into the isometry side (rows of M, or columns
+// when rows_trailing) and the rest: phase 1 computes the decomposition and the
+// spectrum P sorted as NDTensors sorts it; phase 2 writes U and C for k vectors.
+struct FactCore {
+  const double* M;
+  int64_t m, nc;
+  bool rows_trailing;
+  int route;
+  int64_t iso = 0, oth = 0, full = 0;
+  bool gram = false;
+  std::vector<double> P;
+  std::vector<int> order;
+  std::unique_ptr<Scratch> rho, sv, ub, vb;
+  FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
+                                                                         route(r) {
+    iso = rt ? nc : m;
+    oth = rt ? m : nc;
+    full = std::min(m, nc);
+  }
+};
+
+void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
+  // rocSOLVER's dgesvd is ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs
+  // 47 ms), so above a small size the "svd" route also diagonalises the Gram
+  // matrix; its spectrum P = s^2 is then resolved to eps * P_max absolutely.
+  f.gram = f.route == NSMI_ROUTE_EIGEN || f.full > kSvdDirectMax;
+  const int64_t iso = f.iso, oth = f.oth, m = f.m, nc = f.nc;
+  if (f.gram) {
+    f.rho.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
+    Scratch w(ctx, sizeof(double) * size_t(iso));
+    Scratch e(ctx, sizeof(double) * size_t(iso));
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
+      if (!f.rows_trailing)
+        gemm_f64(ctx, gdesc(m, m, nc, false, true), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
+      else
+        gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
+    }
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
+      check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
+                                       f.rho->as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(),
+                                       ctx->d_info),
+                      "rocsolver_dsyevd");
+    }
+    check_info(ctx, "rocsolver_dsyevd");
+    std::vector<double> wh(iso);
+    NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    for (double v : wh)
+      if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
+    f.order.resize(iso);
+    std::iota(f.order.begin(), f.order.end(), 0);
+    if (f.route == NSMI_ROUTE_EIGEN) {
+      // NDTensors eigen: sort by decreasing |lambda|; truncate! zeroes a negative tail
+      std::stable_sort(f.order.begin(), f.order.end(),
+                       [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
+    } else {
+      // svd semantics: P = s^2 >= 0 in decreasing order
+      for (auto& v : wh) v = std::max(v, 0.0);
+      std::stable_sort(f.order.begin(), f.order.end(), [&](int a, int b) { return wh[a] > wh[b]; });
+    }
+    f.P.resize(iso);
+    for (int64_t i = 0; i < iso; ++i) f.P[i] = wh[f.order[i]];
+  } else {
+    // SVD of the column-major view M^T (nc x m): M^T = Ut S Vt  =>  M = Vt^T S Ut^T
+    // row-major readings: buffer "vb" (full x m col-major) = left vectors of M as m x full rows;
+    // buffer "ub" (nc x full col-major) = right vectors of M as full x nc rows.
+    const int64_t full = f.full;
+    Scratch a(ctx, sizeof(double) * size_t(m) * nc);
+    f.sv.reset(new Scratch(ctx, sizeof(double) * size_t(full)));
+    f.ub.reset(new Scratch(ctx, sizeof(double) * size_t(nc) * full));
+    f.vb.reset(new Scratch(ctx, sizeof(double) * size_t(full) * m));
+    Scratch e(ctx, sizeof(double) * size_t(std::max<int64_t>(full, 2)));
+    NSMI_HIP(hipMemcpyAsync(a.p, f.M, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    {
+      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
+      check_rocsolver(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_singular, rocblas_int(nc),
+                                       rocblas_int(m), a.as<double>(), rocblas_int(nc), f.sv->as<double>(),
+                                       f.ub->as<double>(), rocblas_int(nc), f.vb->as<double>(), rocblas_int(full),
+                                       e.as<double>(), rocblas_outofplace, ctx->d_info),
+                      "rocsolver_dgesvd");
+    }
+    check_info(ctx, "rocsolver_dgesvd");
+    std::vector<double> sh(full);
+    NSMI_HIP(hipMemcpyAsync(sh.data(), f.sv->p, sizeof(double) * full, hipMemcpyDeviceToHost, ctx->stream));
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    f.P.resize(full);
+    for (int64_t i = 0; i < full; ++i) {
+      if (!std::isfinite(sh[i])) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite singular value");
+      f.P[i] = sh[i] * sh[i];
+    }
+  }
+}
+
+// U / C for the first k entries of the sorted spectrum (U m x k and C k x nc, or
+// U k x nc and C m x k when rows_trailing), columns of the isometry in the
+// "largest-magnitude entry positive" gauge.
+void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
+  if (k == 0) return;
+  const int64_t m = f.m, nc = f.nc, iso = f.iso, full = f.full;
+  Scratch d_sgn(ctx, sizeof(double) * size_t(std::max<int64_t>(k, 1)));
+  ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(k) * m * nc, 8.0 * (double(m) * nc + double(k) * (m + nc)));
+  if (f.gram) {
+    Scratch d_idx(ctx, sizeof(int) * size_t(k));
+    NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
+    hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.rho->as<double>(), iso, iso, 1,
+                       d_idx.as<int>(), d_sgn.as<double>());
+    NSMI_LAUNCH_CHECK();
+    Scratch Ut(ctx, sizeof(double) * size_t(k) * iso);
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, f.rho->as<double>(), iso,
+                       d_idx.as<int>(), int(k), iso, d_sgn.as<double>(), (const double*)nullptr, Ut.as<double>());
+    NSMI_LAUNCH_CHECK();
+    if (!f.rows_trailing) {
+      // U (m x k) = Ut^T ; C (k x nc) = Ut . M
+      hipLaunchKernelGGL(k_transpose, dim3(blocks(m, 32), blocks(k, 32)), dim3(32, 8), 0, ctx->stream,
+                         Ut.as<double>(), int(k), m, U);
+      NSMI_LAUNCH_CHECK();
+      gemm_f64(ctx, gdesc(k, nc, m, false, false), 1.0, Ut.as<double>(), f.M, 0.0, C, kRoleFact);
+    } else {
+      // U (k x nc) = Ut ; C (m x k) = M . Ut^T
+      NSMI_HIP(hipMemcpyAsync(U, Ut.p, sizeof(double) * k * nc, hipMemcpyDeviceToDevice, ctx->stream));
+      gemm_f64(ctx, gdesc(m, k, nc, false, true), 1.0, f.M, Ut.as<double>(), 0.0, C, kRoleFact);
+    }
+    // the host order vector must outlive the copy above
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  } else {
+    if (!f.rows_trailing) {
+      // isometry = left vectors (columns of the m x full row-major vb)
+      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.vb->as<double>(), m, full, 0,
+                         (const int*)nullptr, d_sgn.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, f.vb->as<double>(), full,
+                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), (const double*)nullptr, U);
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc,
+                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), f.sv->as<double>(), C);
+      NSMI_LAUNCH_CHECK();
+    } else {
+      // isometry = right vectors (rows of the full x nc row-major ub)
+      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc, nc, 1,
+                         (const int*)nullptr, d_sgn.as<double>());
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc,
+                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), (const double*)nullptr, U);
+      NSMI_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, f.vb->as<double>(), full,
+                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), f.sv->as<double>(), C);
+      NSMI_LAUNCH_CHECK();
+    }
+  }
+}
+
+}  // namespace
+}  // namespace nsmi
+
 extern "C" {
 
 #define API_BEGIN try {
@@ -261,158 +422,33 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
     }
     return NSMI_OK;
   }
-  const double* M = static_cast<const double*>(theta->data);
-
-  std::vector<double> P;
-  std::vector<int> order;
-  int64_t k = 0;
+  FactCore fc(static_cast<const double*>(theta->data), m, nc, rows_trailing != 0, route);
+  fact_prepare(ctx, fc);
   double truncerr = 0.0, total = 0.0;
-  nsmi_tensor *U = nullptr, *C = nullptr;
-  Scratch d_sgn(ctx, sizeof(double) * std::max<int64_t>(full, 1));
-  Scratch d_idx(ctx, sizeof(int) * std::max<int64_t>(full, 1));
-
+  std::vector<double> P = fc.P;
+  const int64_t k = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
   int64_t udims[8], cdims[8];
   int ur = 0, cr = 0;
-  auto set_dims = [&](int64_t kk) {
-    ur = cr = 0;
-    if (!rows_trailing) {
-      for (int i = 0; i < n_row_dims; ++i) udims[ur++] = theta->dims[i];
-      udims[ur++] = kk;
-      cdims[cr++] = kk;
-      for (int i = n_row_dims; i < theta->rank; ++i) cdims[cr++] = theta->dims[i];
-    } else {
-      udims[ur++] = kk;
-      for (int i = n_row_dims; i < theta->rank; ++i) udims[ur++] = theta->dims[i];
-      for (int i = 0; i < n_row_dims; ++i) cdims[cr++] = theta->dims[i];
-      cdims[cr++] = kk;
-    }
-  };
-
-  // rocSOLVER's dgesvd is ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs
-  // 47 ms), so above a small size the "svd" route also diagonalises the Gram
-  // matrix; its spectrum P = s^2 is then resolved to eps * P_max absolutely.
-  const bool use_gram = route == NSMI_ROUTE_EIGEN || full > kSvdDirectMax;
-  if (use_gram) {
-    // rho over the isometry side (iso x iso)
-    Scratch rho(ctx, sizeof(double) * size_t(iso) * iso);
-    Scratch w(ctx, sizeof(double) * size_t(iso));
-    Scratch e(ctx, sizeof(double) * size_t(iso));
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
-      if (!rows_trailing)
-        gemm_f64(ctx, gdesc(m, m, nc, false, true), 1.0, M, M, 0.0, rho.as<double>(), kRoleFact);
-      else
-        gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, M, M, 0.0, rho.as<double>(), kRoleFact);
-    }
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
-      check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
-                                       rho.as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(), ctx->d_info),
-                      "rocsolver_dsyevd");
-    }
-    check_info(ctx, "rocsolver_dsyevd");
-    std::vector<double> wh(iso);
-    NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
-    NSMI_HIP(hipStreamSynchronize(ctx->stream));
-    for (double v : wh)
-      if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
-    order.resize(iso);
-    std::iota(order.begin(), order.end(), 0);
-    if (route == NSMI_ROUTE_EIGEN) {
-      // NDTensors eigen: sort by decreasing |lambda|; truncate! zeroes a negative tail
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
-    } else {
-      // svd semantics: P = s^2 >= 0 in decreasing order
-      for (auto& v : wh) v = std::max(v, 0.0);
-      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wh[a] > wh[b]; });
-    }
-    P.resize(iso);
-    for (int64_t i = 0; i < iso; ++i) P[i] = wh[order[i]];
-    k = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
-    NSMI_HIP(hipMemcpyAsync(d_idx.p, order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
-    // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
-    hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, rho.as<double>(), iso, iso, 1,
-                       d_idx.as<int>(), d_sgn.as<double>());
-    NSMI_LAUNCH_CHECK();
-    Scratch Ut(ctx, sizeof(double) * size_t(k) * iso);
-    hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, rho.as<double>(), iso,
-                       d_idx.as<int>(), int(k), iso, d_sgn.as<double>(), (const double*)nullptr, Ut.as<double>());
-    NSMI_LAUNCH_CHECK();
-    set_dims(k);
-    U = tensor_new(ctx, NSMI_F64, ur, udims);
-    C = tensor_new(ctx, NSMI_F64, cr, cdims);
-    ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(k) * m * nc, 8.0 * (double(m) * nc + double(k) * (m + nc)));
-    if (!rows_trailing) {
-      // U (m x k) = Ut^T ; C (k x nc) = Ut . M
-      hipLaunchKernelGGL(k_transpose, dim3(blocks(m, 32), blocks(k, 32)), dim3(32, 8), 0, ctx->stream, Ut.as<double>(),
-                         int(k), m, static_cast<double*>(U->data));
-      NSMI_LAUNCH_CHECK();
-      gemm_f64(ctx, gdesc(k, nc, m, false, false), 1.0, Ut.as<double>(), M, 0.0, static_cast<double*>(C->data),
-               kRoleFact);
-    } else {
-      // U (k x nc) = Ut ; C (m x k) = M . Ut^T
-      NSMI_HIP(hipMemcpyAsync(U->data, Ut.p, sizeof(double) * k * nc, hipMemcpyDeviceToDevice, ctx->stream));
-      gemm_f64(ctx, gdesc(m, k, nc, false, true), 1.0, M, Ut.as<double>(), 0.0, static_cast<double*>(C->data),
-               kRoleFact);
-    }
+  if (!rows_trailing) {
+    for (int i = 0; i < n_row_dims; ++i) udims[ur++] = theta->dims[i];
+    udims[ur++] = k;
+    cdims[cr++] = k;
+    for (int i = n_row_dims; i < theta->rank; ++i) cdims[cr++] = theta->dims[i];
   } else {
-    // SVD of the column-major view M^T (nc x m): M^T = Ut S Vt  =>  M = Vt^T S Ut^T
-    // row-major readings: buffer "vt" (full x m col-major) = left vectors of M as m x full rows;
-    // buffer "u" (nc x full col-major) = right vectors of M as full x nc rows.
-    Scratch a(ctx, sizeof(double) * size_t(m) * nc);
-    Scratch s(ctx, sizeof(double) * size_t(full));
-    Scratch ub(ctx, sizeof(double) * size_t(nc) * full);
-    Scratch vb(ctx, sizeof(double) * size_t(full) * m);
-    Scratch e(ctx, sizeof(double) * size_t(std::max<int64_t>(full, 2)));
-    NSMI_HIP(hipMemcpyAsync(a.p, M, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
-      check_rocsolver(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_singular, rocblas_int(nc),
-                                       rocblas_int(m), a.as<double>(), rocblas_int(nc), s.as<double>(),
-                                       ub.as<double>(), rocblas_int(nc), vb.as<double>(), rocblas_int(full),
-                                       e.as<double>(), rocblas_outofplace, ctx->d_info),
-                      "rocsolver_dgesvd");
-    }
-    check_info(ctx, "rocsolver_dgesvd");
-    std::vector<double> sh(full);
-    NSMI_HIP(hipMemcpyAsync(sh.data(), s.p, sizeof(double) * full, hipMemcpyDeviceToHost, ctx->stream));
-    NSMI_HIP(hipStreamSynchronize(ctx->stream));
-    P.resize(full);
-    for (int64_t i = 0; i < full; ++i) {
-      if (!std::isfinite(sh[i])) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite singular value");
-      P[i] = sh[i] * sh[i];
-    }
-    k = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
-    set_dims(k);
-    U = tensor_new(ctx, NSMI_F64, ur, udims);
+    udims[ur++] = k;
+    for (int i = n_row_dims; i < theta->rank; ++i) udims[ur++] = theta->dims[i];
+    for (int i = 0; i < n_row_dims; ++i) cdims[cr++] = theta->dims[i];
+    cdims[cr++] = k;
+  }
+  nsmi_tensor* U = tensor_new(ctx, NSMI_F64, ur, udims);
+  nsmi_tensor* C = nullptr;
+  try {
     C = tensor_new(ctx, NSMI_F64, cr, cdims);
-    if (!rows_trailing) {
-      // isometry = left vectors (columns of the m x full row-major vb)
-      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, vb.as<double>(), m, full, 0,
-                         (const int*)nullptr, d_sgn.as<double>());
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, vb.as<double>(), full,
-                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), (const double*)nullptr,
-                         static_cast<double*>(U->data));
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, ub.as<double>(), nc,
-                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), s.as<double>(),
-                         static_cast<double*>(C->data));
-      NSMI_LAUNCH_CHECK();
-    } else {
-      // isometry = right vectors (rows of the full x nc row-major ub)
-      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, ub.as<double>(), nc, nc, 1,
-                         (const int*)nullptr, d_sgn.as<double>());
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, ub.as<double>(), nc,
-                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), (const double*)nullptr,
-                         static_cast<double*>(U->data));
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, vb.as<double>(), full,
-                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), s.as<double>(),
-                         static_cast<double*>(C->data));
-      NSMI_LAUNCH_CHECK();
-    }
+    fact_emit(ctx, fc, k, static_cast<double*>(U->data), static_cast<double*>(C->data));
+  } catch (...) {
+    tensor_delete(ctx, U);
+    if (C) tensor_delete(ctx, C);
+    throw;
   }
   *Uout = U;
   *Cout = C;
@@ -522,6 +558,113 @@ int nsmi_qr(nsmi_ctx* ctx, const nsmi_tensor* A, int n_row_dims, int rows_traili
   *Qout = Q;
   *Rout = R;
   API_END(ctx)
+}
+
+// Block-diagonal (quantum-number) factorisation: every sector matrix M[q] is
+// decomposed as nsmi_factorize would, the sector spectra are merged and truncated
+// together (NDTensors truncate! on the concatenated, sorted spectrum), and each
+// sector keeps the entries above the cut value docut (NDTensors' block-sparse
+// svd/eigen): the midpoint between the last kept and first discarded values, moved
+// up by 1e-3 of the last kept value when the two are nearly degenerate, so a
+// near-degenerate pair is kept or dropped together.
+int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const int64_t* offsets, const int64_t* rows,
+                           const int64_t* cols, int rows_trailing, const nsmi_trunc* trunc, nsmi_tensor** Uall,
+                           nsmi_tensor** Call, int64_t* kept, nsmi_trunc_info* info) {
+  if (!ctx || !Mall || n < 0 || (n > 0 && (!offsets || !rows || !cols || !kept)) || !trunc || !Uall || !Call)
+    return NSMI_ERR_INVALID;
+  std::vector<nsmi_tensor*> made;
+  API_BEGIN
+  *Uall = nullptr;
+  *Call = nullptr;
+  require(Mall->dtype == NSMI_F64, "factorize_sectors: sector matrices are FP64");
+  for (int q = 0; q < n; ++q) {
+    kept[q] = 0;
+    require(rows[q] >= 1 && cols[q] >= 1 && offsets[q] >= 0 && offsets[q] + rows[q] * cols[q] <= Mall->numel,
+            "factorize_sectors: sector matrix outside the buffer");
+  }
+  int route = trunc->route;
+  if (route == NSMI_ROUTE_AUTO) route = (trunc->cutoff <= 1e-12) ? NSMI_ROUTE_SVD : NSMI_ROUTE_EIGEN;
+  require(route == NSMI_ROUTE_SVD || route == NSMI_ROUTE_EIGEN, "factorize_sectors: unknown route");
+  std::vector<std::unique_ptr<FactCore>> fc;
+  int64_t full = 0;
+  for (int q = 0; q < n; ++q) {
+    fc.emplace_back(new FactCore(static_cast<const double*>(Mall->data) + offsets[q], rows[q], cols[q],
+                                 rows_trailing != 0, route));
+    fact_prepare(ctx, *fc.back());
+    full += fc.back()->full;
+  }
+  auto key = [&](double v) { return route == NSMI_ROUTE_EIGEN ? std::fabs(v) : v; };
+  struct Entry {
+    double key, value;
+    int q, i;
+  };
+  std::vector<Entry> all;
+  for (int q = 0; q < n; ++q)
+    for (size_t i = 0; i < fc[q]->P.size(); ++i) all.push_back(Entry{key(fc[q]->P[i]), fc[q]->P[i], q, int(i)});
+  std::stable_sort(all.begin(), all.end(), [](const Entry& a, const Entry& b) { return a.key > b.key; });
+  std::vector<double> P(all.size());
+  for (size_t i = 0; i < all.size(); ++i) P[i] = all[i].value;
+  const int64_t maxdim = trunc->maxdim > 0 ? std::min<int64_t>(trunc->maxdim, full) : full;
+  const int64_t mindim = trunc->mindim > 0 ? trunc->mindim : 1;
+  double truncerr = 0.0, total = 0.0;
+  const int64_t origm = int64_t(P.size());
+  const int64_t nk = truncate_spectrum(P, mindim, maxdim, trunc->cutoff, &truncerr, &total);
+  int64_t sum = 0;
+  if (nk < origm) {
+    double docut = (P[nk - 1] + P[nk]) / 2;
+    if (std::fabs(P[nk - 1] - P[nk]) < 1e-3 * P[nk - 1]) docut += 1e-3 * P[nk - 1];
+    for (size_t i = 0; i < all.size(); ++i)
+      if (P[i] > docut) ++kept[all[i].q];
+  } else {
+    for (int q = 0; q < n; ++q) kept[q] = int64_t(fc[q]->P.size());
+  }
+  for (int q = 0; q < n; ++q) sum += kept[q];
+  if (sum == 0 && !all.empty()) {  // keep at least the largest entry
+    kept[all[0].q] = 1;
+    sum = 1;
+  }
+  // U_q (rows x k, or k x cols when rows_trailing) and C_q concatenated in sector order
+  int64_t usz = 0, csz = 0;
+  for (int q = 0; q < n; ++q) {
+    usz += kept[q] * (rows_trailing ? cols[q] : rows[q]);
+    csz += kept[q] * (rows_trailing ? rows[q] : cols[q]);
+  }
+  const int64_t ud[1] = {std::max<int64_t>(usz, 1)}, cd[1] = {std::max<int64_t>(csz, 1)};
+  nsmi_tensor* Ut = tensor_new(ctx, NSMI_F64, 1, ud);
+  made.push_back(Ut);
+  nsmi_tensor* Ct = tensor_new(ctx, NSMI_F64, 1, cd);
+  made.push_back(Ct);
+  int64_t uo = 0, co = 0;
+  for (int q = 0; q < n; ++q) {
+    const int64_t k = kept[q];
+    if (k == 0) continue;
+    fact_emit(ctx, *fc[q], k, static_cast<double*>(Ut->data) + uo, static_cast<double*>(Ct->data) + co);
+    uo += k * (rows_trailing ? cols[q] : rows[q]);
+    co += k * (rows_trailing ? rows[q] : cols[q]);
+  }
+  if (info) {
+    info->kept = sum;
+    info->full = full;
+    info->truncerr = truncerr;
+    info->total = total;
+    info->route = route;
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  *Uall = Ut;
+  *Call = Ct;
+  made.clear();
+  }
+  catch (const nsmi::Error& e) {
+    for (auto* t : made) tensor_delete(ctx, t);
+    ctx->err = e.what();
+    return e.code;
+  }
+  catch (const std::exception& e) {
+    for (auto* t : made) tensor_delete(ctx, t);
+    ctx->err = std::string("internal: ") + e.what();
+    return NSMI_ERR_INVALID;
+  }
+  return NSMI_OK;
 }
 
 }  // extern "C"

@@ -221,6 +221,13 @@ int64_t net_size(const NetProgram* p);
 bool net_is_complex(const NetProgram* p);
 nsmi_heff* heff_wrap_net(NetProgram* p);  // region.hip: an nsmi_heff that runs the program
 
+// block-sparse (QN) block programs (blocksparse.hip)
+struct BsProgram;
+void bs_run(nsmi_ctx* ctx, BsProgram* p, const double* x, double* y);
+void bs_release(nsmi_ctx* ctx, BsProgram* p);
+nsmi_heff* heff_wrap_bs(BsProgram* p, int64_t n);  // region.hip; n < 0: not a square operator
+BsProgram* heff_bs_program(nsmi_heff* op);
+
 // H_eff operator access for the Krylov solvers (krylov.cpp)
 void heff_apply_dev(nsmi_ctx* ctx, nsmi_heff* op, const double* x, double* y);
 int64_t heff_size(const nsmi_heff* op);

@@ -36,6 +36,7 @@ struct nsmi_heff {
   double flops_gemm1 = 0, flops_gemm2 = 0, flops_mix = 0;
   int64_t n = 0;  // elements of theta
   nsmi::NetProgram* net = nullptr;  // tree regions: a general contraction program instead of the fixed form
+  nsmi::BsProgram* bs = nullptr;    // quantum-number regions: a block program
 };
 
 namespace nsmi {
@@ -323,6 +324,10 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     net_apply(ctx, op->net, x, y);
     return;
   }
+  if (op->bs) {
+    bs_run(ctx, op->bs, x, y);
+    return;
+  }
   const int64_t M1 = op->cl * op->D0, K1 = op->cl, N1 = op->dphys * op->cr;
   const int64_t M2 = op->cl * op->dphys, K2 = op->D2 * op->cr, N2 = op->cr;
   GemmDesc g1 = desc(M1, N1, K1, false, false);
@@ -408,6 +413,7 @@ static void heff_release(nsmi_ctx* ctx, nsmi_heff* op) {
                   op->xThi, op->xTlo, op->T3hi, op->T3lo, op->T1f};
   for (void* b : bufs) scratch_release(ctx, b);
   net_release(ctx, op->net);
+  bs_release(ctx, op->bs);
   delete op;
 }
 
@@ -421,6 +427,16 @@ nsmi_heff* heff_wrap_net(NetProgram* p) {
   op->n = net_size(p);
   return op;
 }
+nsmi_heff* heff_wrap_bs(BsProgram* p, int64_t n) {
+  nsmi_heff* op = new nsmi_heff();
+  op->bs = p;
+  op->nsite = 0;
+  op->precision = NSMI_PREC_FP64;
+  op->cplx = false;
+  op->n = n;
+  return op;
+}
+BsProgram* heff_bs_program(nsmi_heff* op) { return op ? op->bs : nullptr; }
 }  // namespace nsmi
 
 extern "C" {

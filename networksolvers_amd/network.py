@@ -221,7 +221,9 @@ def expect_mpo(psi: TTN, H: TTNO, precision="fp64") -> float:
     """<psi|H|psi> / <psi|psi> by a full left-to-right environment pass in FP64
     (on a tree: the Rayleigh quotient at an orthogonality centre)."""
     if not isinstance(psi, TTN):
-        from . import tree
+        from . import qn, tree
+        if isinstance(psi, qn.QNMPS):
+            return qn.expect_mpo(psi, H)
         return tree.expect_mpo(psi, H)
     ctx = H.ctx
     E = ctx.upload(np.ones((1, 1, 1), dtype=psi.dtype))

@@ -1,0 +1,685 @@
+"""U(1) quantum-number (block-sparse) chains on the device.
+
+The reference runs the same sweeps on QN-conserving ITensors (siteinds with
+conserve_qns=true; README.md QN timings, examples/dmrg.jl conserve_qns): every
+tensor is a set of dense blocks, one per charge-consistent combination of index
+sectors, and every contraction is a sum over compatible block pairs
+(operator_map.jl:3-10 "per compatible block pair").  Here the host works out the
+block bookkeeping and the device executes it as block programs
+(``nsmi_bsop_create``): grouped GEMM launches in which each output block sums its
+segments in one pass, grouped linear combinations for the MPO mixing, grouped
+copies for sector assembly.  The truncated split is ``nsmi_factorize_sectors``.
+
+Conventions.  Charges are integers (2 Sz for spin states).  A site tensor
+A[l, s, r] has arrows (in, in, out): q_l + q_s = q_r; the two-site tensor
+theta[l, s1, s2, r]: q_l + q_s1 + q_s2 = q_r; an environment E[bra, w, ket]:
+q_bra - q_ket = c_w, where c_w is the MPO link charge, propagated from the left
+boundary by c_wr = c_wl + q_s' - q_s over the MPO's nonzeros.  Site and MPO
+indices have one sector per basis state, so every block is a (link x link)
+matrix, stored row-major, blocks concatenated in one flat FP64 buffer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import itertools
+
+import numpy as np
+
+from . import _lib, kernels
+from .device import Context, DeviceTensor
+from .region_plans import path_graph
+
+
+# ------------------------------------------------------------------ indices
+class QNIndex:
+    """Sectors (charge, dim) in a fixed order; dense offsets follow that order."""
+
+    def __init__(self, sectors):
+        self.q = [int(a) for a, n in sectors]
+        self.n = [int(n) for a, n in sectors]
+        if any(n <= 0 for n in self.n):
+            raise ValueError("sector dimensions must be positive")
+        self.off = [int(x) for x in np.concatenate([[0], np.cumsum(self.n)])[:-1]] if self.n else []
+        self.dim = int(sum(self.n))
+        self._by_q = {}
+        for i, q in enumerate(self.q):
+            self._by_q.setdefault(q, []).append(i)
+
+    def find(self, q):
+        """The sector with charge q (link indices have unique charges), or None."""
+        ids = self._by_q.get(int(q))
+        return ids[0] if ids else None
+
+    def __len__(self):
+        return len(self.q)
+
+    def __repr__(self):
+        return f"QNIndex({list(zip(self.q, self.n))})"
+
+
+def spin_site_index(d):
+    """Spin S = (d-1)/2 basis m = S, S-1, ..., -S ("Up" first) with charge 2m."""
+    return QNIndex([(int(round(2 * ((d - 1) / 2 - i))), 1) for i in range(d)])
+
+
+# ------------------------------------------------------------------ tensors
+class QNTensor:
+    """A block-sparse tensor: ``keys`` (tuples of sector ids) in storage order, each
+    block dense with the sector dims, all in one flat device buffer."""
+
+    def __init__(self, ctx: Context, indices, dirs, flux=0, keys=None, data: DeviceTensor | None = None):
+        self.ctx = ctx
+        self.indices = list(indices)
+        self.dirs = list(dirs)
+        self.flux = int(flux)
+        if keys is None:
+            keys = [k for k in itertools.product(*[range(len(ix)) for ix in self.indices])
+                    if sum(d * ix.q[i] for d, ix, i in zip(self.dirs, self.indices, k)) == self.flux]
+        self.keys = [tuple(k) for k in keys]
+        self.pos = {k: i for i, k in enumerate(self.keys)}
+        self.shapes = [tuple(ix.n[i] for ix, i in zip(self.indices, k)) for k in self.keys]
+        sizes = [int(np.prod(s)) for s in self.shapes]
+        self.offs = [int(x) for x in np.concatenate([[0], np.cumsum(sizes)])[:-1]] if sizes else []
+        self.size = int(sum(sizes))
+        if data is None:
+            data = ctx.upload(np.zeros(max(self.size, 1)))
+        self.data = data
+
+    def rows_cols(self, key):
+        s = self.shapes[self.pos[key]]
+        return s[0], int(np.prod(s[1:]))
+
+    def block_off(self, key):
+        return self.offs[self.pos[key]]
+
+    def has(self, key):
+        return key in self.pos
+
+    @classmethod
+    def from_dense(cls, ctx, arr, indices, dirs, flux=0, atol=0.0):
+        t = cls(ctx, indices, dirs, flux)
+        flat = np.zeros(max(t.size, 1))
+        covered = np.zeros(arr.shape, dtype=bool)
+        for k, off, shp in zip(t.keys, t.offs, t.shapes):
+            sl = tuple(slice(ix.off[i], ix.off[i] + ix.n[i]) for ix, i in zip(t.indices, k))
+            flat[off:off + int(np.prod(shp))] = np.asarray(arr[sl], dtype=np.float64).reshape(-1)
+            covered[sl] = True
+        if np.any(np.abs(arr[~covered]) > atol):
+            raise ValueError("dense array has weight outside the charge-allowed blocks")
+        t.data = ctx.upload(flat)
+        return t
+
+    def to_dense(self):
+        flat = self.data.numpy()
+        out = np.zeros([ix.dim for ix in self.indices])
+        for k, off, shp in zip(self.keys, self.offs, self.shapes):
+            sl = tuple(slice(ix.off[i], ix.off[i] + ix.n[i]) for ix, i in zip(self.indices, k))
+            out[sl] = flat[off:off + int(np.prod(shp))].reshape(shp)
+        return out
+
+    def copy(self):
+        return QNTensor(self.ctx, self.indices, self.dirs, self.flux, self.keys, self.data.copy())
+
+    def norm(self):
+        return self.data.norm()
+
+    @property
+    def h(self):
+        """The flat buffer's handle: Krylov solvers act on the block vector directly."""
+        return self.data.h
+
+    @property
+    def shape(self):
+        return (self.size,)
+
+    @property
+    def dtype(self):
+        return np.dtype(np.float64)
+
+    def scale_(self, a):
+        self.data.scale_(a)
+        return self
+
+
+# ------------------------------------------------------------ block programs
+class _Program:
+    """Collects the steps of one block program (buffer ids: 0 = x, 1 = y, 2.. =
+    operands, then scratch)."""
+
+    def __init__(self):
+        self.operands = []
+        self.scratch = []
+        self.steps = []
+        self.gitems, self.segs, self.alpha = [], [], []
+        self.litems, self.terms, self.coeff = [], [], []
+        self.citems = []
+
+    def operand(self, t: DeviceTensor):
+        for i, o in enumerate(self.operands):
+            if o is t:
+                return 2 + i
+        self.operands.append(t)
+        return 1 + len(self.operands)
+
+    def scratch_buf(self, n):
+        self.scratch.append(int(n))
+        return None  # resolved in build (ids after the operands)
+
+    def gemm(self, ta, tb, items):
+        """items: [(c_buf, c_off, ldc, M, N, [(a_buf, a_off, lda, b_buf, b_off, ldb, K, alpha), ...])]"""
+        first = len(self.gitems)
+        for c_buf, c_off, ldc, M, N, sg in items:
+            self.gitems.append((c_buf, c_off, ldc, M, N, len(self.segs), len(sg)))
+            for a_buf, a_off, lda, b_buf, b_off, ldb, K, al in sg:
+                self.segs.append((a_buf, a_off, lda, b_buf, b_off, ldb, K))
+                self.alpha.append(al)
+        self.steps.append((0, int(ta), int(tb), first, len(items), 0))
+
+    def lincomb(self, items):
+        """items: [(out_buf, out_off, n, [(src_buf, src_off, coeff), ...])]"""
+        first = len(self.litems)
+        for ob, oo, n, tl in items:
+            self.litems.append((ob, oo, n, len(self.terms), len(tl)))
+            for sb, so, c in tl:
+                self.terms.append((sb, so))
+                self.coeff.append(c)
+        self.steps.append((1, 0, 0, first, len(items), 0))
+
+    def copy(self, items):
+        """items: [(src_buf, src_off, src_ld, dst_buf, dst_off, dst_ld, rows, cols)]"""
+        first = len(self.citems)
+        self.citems.extend(items)
+        self.steps.append((2, 0, 0, first, len(items), 0))
+
+    def sid(self, j):
+        """Buffer id of scratch buffer j (valid once all operands are registered)."""
+        return 2 + len(self.operands) + j
+
+    def build(self, ctx: Context, x_elems, y_elems):
+        def arr(rows, w, dt=C.c_int64):
+            flat = [v for r in rows for v in r] if rows else [0] * w
+            return (dt * max(len(flat), 1))(*flat)
+
+        ops = (C.c_void_p * max(len(self.operands), 1))(*[o.h.value for o in self.operands])
+        scr = (C.c_int64 * max(len(self.scratch), 1))(*self.scratch)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.nsmi_bsop_create(
+            ctx.h, len(self.operands), ops, len(self.scratch), scr, int(x_elems), int(y_elems), len(self.steps),
+            arr(self.steps, 6), arr(self.gitems, 7), arr(self.segs, 7),
+            (C.c_double * max(len(self.alpha), 1))(*self.alpha), arr(self.litems, 5), arr(self.terms, 2),
+            (C.c_double * max(len(self.coeff), 1))(*self.coeff), arr(self.citems, 8), C.byref(h)))
+        return BlockProgram(ctx, h, self.operands)
+
+
+class BlockProgram(kernels._OperatorBase):
+    """A built block program; an effective Hamiltonian when x and y have one size."""
+
+    def __init__(self, ctx, h, keep):
+        self.ctx = ctx
+        self.h = h
+        self._keep = tuple(keep)
+        self.precision = "fp64"
+
+    def run(self, x: DeviceTensor | None, y: DeviceTensor | None):
+        self.ctx.check(self.ctx.lib.nsmi_bsop_run(self.ctx.h, self.h, x.h if x is not None else None,
+                                                  y.h if y is not None else None))
+
+
+def _run_once(prog: _Program, ctx, x, y):
+    op = prog.build(ctx, x.size if x is not None else 0, y.size if y is not None else 0)
+    try:
+        op.run(x.data if x is not None else None, y.data if y is not None else None)
+    finally:
+        op.close()
+
+
+# ---------------------------------------------------------------- operators
+class QNMPO:
+    """Chain MPO W[wl, wr, s', s] (host, FP64) with its MPO link charges inferred
+    by propagation from the left boundary."""
+
+    def __init__(self, ctx: Context, W, site: QNIndex):
+        self.ctx = ctx
+        self.W = [np.asarray(w, dtype=np.float64) for w in W]
+        self.site = site
+        qs = site.q
+        N = len(self.W)
+        self.c = [np.zeros(self.W[0].shape[0], dtype=np.int64)]
+        for i, w in enumerate(self.W):
+            cl = self.c[-1]
+            cr = np.full(w.shape[1], np.iinfo(np.int64).min, dtype=np.int64)
+            for a, b, sp, s in zip(*np.nonzero(w)):
+                v = cl[a] + qs[sp] - qs[s]
+                if cr[b] == np.iinfo(np.int64).min:
+                    cr[b] = v
+                elif cr[b] != v:
+                    raise ValueError(f"MPO site {i} does not conserve the charge")
+            cr[cr == np.iinfo(np.int64).min] = 0
+            self.c.append(cr)
+        if N and any(self.c[-1] != 0):
+            raise ValueError("MPO right boundary must carry charge 0")
+        self.links = [QNIndex([(int(q), 1) for q in c]) for c in self.c]
+
+    def __len__(self):
+        return len(self.W)
+
+
+def qn_product_mps(ctx, states, site: QNIndex):
+    """Product state with site i in basis state states[i]: link charges accumulate."""
+    links = [QNIndex([(0, 1)])]
+    q = 0
+    for s in states:
+        q += site.q[s]
+        links.append(QNIndex([(q, 1)]))
+    tensors = []
+    for i, s in enumerate(states):
+        A = QNTensor(ctx, [links[i], site, links[i + 1]], [1, 1, -1], 0, keys=[(0, s, 0)])
+        A.data = ctx.upload(np.ones(1))
+        tensors.append(A)
+    return QNMPS(ctx, tensors, links, site)
+
+
+class QNMPS:
+    """QN chain state; every site tensor is an isometry for a product state, so the
+    state starts with a known (trivial) gauge."""
+
+    def __init__(self, ctx, tensors, links, site, ortho_region=None):
+        self.ctx = ctx
+        self.tensors = list(tensors)
+        self.links = list(links)
+        self.site = site
+        self.graph = path_graph(len(tensors))
+        self.ortho_region = ortho_region if ortho_region is not None else [len(tensors) - 1]
+        self.clock = 0
+        self.stamp = [0] * len(tensors)
+
+    def __len__(self):
+        return len(self.tensors)
+
+    def __getitem__(self, i):
+        return self.tensors[i]
+
+    def __setitem__(self, i, t):
+        self.tensors[i] = t
+        self.clock += 1
+        self.stamp[i] = self.clock
+
+    @property
+    def dtype(self):
+        return np.dtype(np.float64)
+
+    def linkdims(self):
+        return [self.links[i + 1].dim for i in range(len(self) - 1)]
+
+    def maxlinkdim(self):
+        return max(self.linkdims(), default=1)
+
+    def to_arrays(self):
+        return [t.to_dense() for t in self.tensors]
+
+    def copy(self):
+        c = QNMPS(self.ctx, [t.copy() for t in self.tensors], self.links, self.site, list(self.ortho_region))
+        return c
+
+
+def _boundary_env(ctx, link: QNIndex, mpo_link: QNIndex):
+    E = QNTensor(ctx, [link, mpo_link, link], [1, -1, -1], 0)
+    E.data = ctx.upload(np.ones(max(E.size, 1)))
+    return E
+
+
+def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
+    """E'[b', w', b] = sum conj(A[a', s', b']) E[a', w, a] W[w, w', s', s] A[a, s, b]
+    (extracter.jl:14 -> ProjTTN environment) as a block program: X = E.A (GEMM),
+    Y = W-mix (LINCOMB), E' = A^T.Y (GEMM with segments over (a', s'))."""
+    l, site, r = A.indices
+    mw_r = QNIndex([(int(q), 1) for q in cr])
+    out = QNTensor(ctx, [r, mw_r, r], [1, -1, -1], 0)
+    p = _Program()
+    xE, xA = p.operand(E.data), 0
+    X, X_off, xs = {}, 0, []
+    for (a, s, b) in A.keys:
+        for w in range(len(cl)):
+            ap = l.find(l.q[a] + cl[w])
+            if ap is None or not E.has((ap, w, a)):
+                continue
+            X[(ap, w, s, b)] = X_off
+            xs.append(((ap, w, s, b), E.block_off((ap, w, a)), A.block_off((a, s, b)), l.n[ap], l.n[a], r.n[b]))
+            X_off += l.n[ap] * r.n[b]
+    Y, Y_off, ys = {}, 0, {}
+    for (ap, w, s, b), _ in X.items():
+        for wp in range(W.shape[1]):
+            for sp in range(W.shape[2]):
+                c = W[w, wp, sp, s]
+                if c != 0.0:
+                    ys.setdefault((ap, sp, wp, b), []).append(((ap, w, s, b), c))
+    for k in ys:
+        Y[k] = Y_off
+        Y_off += l.n[k[0]] * r.n[k[3]]
+    p.scratch_buf(X_off)
+    p.scratch_buf(Y_off)
+    sX, sY = p.sid(0), p.sid(1)
+    p.gemm(False, False, [(sX, X[k], nb, na_p, nb, [(xE, eo, na, xA, ao, nb, na, 1.0)])
+                          for (k, eo, ao, na_p, na, nb) in xs])
+    p.lincomb([(sY, Y[k], l.n[k[0]] * r.n[k[3]], [(sX, X[xk], c) for xk, c in tl]) for k, tl in ys.items()])
+    items = []
+    for (bp, wp, b) in out.keys:
+        sg = []
+        for (ap, sp, bpp) in A.keys:
+            if bpp != bp or (ap, sp, wp, b) not in Y:
+                continue
+            sg.append((xA, A.block_off((ap, sp, bp)), r.n[bp], sY, Y[(ap, sp, wp, b)], r.n[b], l.n[ap], 1.0))
+        items.append((1, out.block_off((bp, wp, b)), r.n[b], r.n[bp], r.n[b], sg))
+    p.gemm(True, False, items)
+    _run_once(p, ctx, A, out)
+    return out
+
+
+def env_right(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
+    """E'[a', w, a] = sum conj(A[a', s', b']) E[b', v, b] W[w, v, s', s] A[a, s, b]:
+    X = A.E^T (GEMM), Y = W-mix (LINCOMB), E' = A.Y^T (GEMM, segments over (s', b'))."""
+    l, site, r = A.indices
+    mw_l = QNIndex([(int(q), 1) for q in cl])
+    out = QNTensor(ctx, [l, mw_l, l], [1, -1, -1], 0)
+    p = _Program()
+    xE, xA = p.operand(E.data), 0
+    X, X_off, xs = {}, 0, []
+    for (a, s, b) in A.keys:
+        for v in range(len(cr)):
+            bp = r.find(r.q[b] + cr[v])
+            if bp is None or not E.has((bp, v, b)):
+                continue
+            X[(a, s, v, bp)] = X_off
+            xs.append(((a, s, v, bp), A.block_off((a, s, b)), E.block_off((bp, v, b)), l.n[a], r.n[b], r.n[bp]))
+            X_off += l.n[a] * r.n[bp]
+    ys = {}
+    for (a, s, v, bp) in X:
+        for w in range(W.shape[0]):
+            for sp in range(W.shape[2]):
+                c = W[w, v, sp, s]
+                if c != 0.0:
+                    ys.setdefault((a, w, sp, bp), []).append(((a, s, v, bp), c))
+    Y, Y_off = {}, 0
+    for k in ys:
+        Y[k] = Y_off
+        Y_off += l.n[k[0]] * r.n[k[3]]
+    p.scratch_buf(X_off)
+    p.scratch_buf(Y_off)
+    sX, sY = p.sid(0), p.sid(1)
+    p.gemm(False, True, [(sX, X[k], nbp, na, nbp, [(xA, ao, nb, xE, eo, nb, nb, 1.0)])
+                         for (k, ao, eo, na, nb, nbp) in xs])
+    p.lincomb([(sY, Y[k], l.n[k[0]] * r.n[k[3]], [(sX, X[xk], c) for xk, c in tl]) for k, tl in ys.items()])
+    items = []
+    for (ap, w, a) in out.keys:
+        sg = []
+        for (app, sp, bp) in A.keys:
+            if app != ap or (a, w, sp, bp) not in Y:
+                continue
+            sg.append((xA, A.block_off((ap, sp, bp)), r.n[bp], sY, Y[(a, w, sp, bp)], r.n[bp], r.n[bp], 1.0))
+        items.append((1, out.block_off((ap, w, a)), l.n[a], l.n[ap], l.n[a], sg))
+    p.gemm(False, True, items)
+    _run_once(p, ctx, A, out)
+    return out
+
+
+def theta(A: QNTensor, B: QNTensor, ctx) -> QNTensor:
+    """theta[l, s1, s2, r] = sum_m A[l, s1, m] B[m, s2, r] (extracter.jl:7) over all
+    charge-allowed blocks of the outer links (a block without a contributing pair
+    is zero)."""
+    l, s1i, m = A.indices
+    _, s2i, r = B.indices
+    th = QNTensor(ctx, [l, s1i, s2i, r], [1, 1, 1, -1], 0)
+    p = _Program()
+    xB = p.operand(B.data)
+    items = []
+    for (a, s1, s2, b) in th.keys:
+        mm = m.find(l.q[a] + s1i.q[s1])
+        sg = []
+        if mm is not None and A.has((a, s1, mm)) and B.has((mm, s2, b)):
+            sg.append((0, A.block_off((a, s1, mm)), m.n[mm], xB, B.block_off((mm, s2, b)), r.n[b], m.n[mm], 1.0))
+        items.append((1, th.block_off((a, s1, s2, b)), r.n[b], l.n[a], r.n[b], sg))
+    p.gemm(False, False, items)
+    _run_once(p, ctx, A, th)
+    return th
+
+
+def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx) -> BlockProgram:
+    """Two-site effective Hamiltonian (operator_map.jl:3-10) on theta's block space:
+    T1 = L.theta (GEMM), T3 = (W1 W2)-mix (LINCOMB), y = T3.R^T (GEMM, segments over
+    the right MPO link)."""
+    l, s1i, s2i, r = th.indices
+    W12 = np.einsum("wmas,mvbt->wstabv", W1, W2)   # [w, s1, s2, s1', s2', v]
+    nz = {}
+    for w, s1, s2, a1, a2, v in zip(*np.nonzero(W12)):
+        nz.setdefault((w, s1, s2), []).append((a1, a2, v, W12[w, s1, s2, a1, a2, v]))
+    p = _Program()
+    xL, xR = p.operand(L.data), p.operand(R.data)
+    T1, T1_off, t1 = {}, 0, []
+    for (a, s1, s2, b) in th.keys:
+        for w in range(len(cL)):
+            ap = l.find(l.q[a] + cL[w])
+            if ap is None or not L.has((ap, w, a)):
+                continue
+            T1[(ap, w, s1, s2, b)] = T1_off
+            t1.append(((ap, w, s1, s2, b), L.block_off((ap, w, a)), th.block_off((a, s1, s2, b)), l.n[ap], l.n[a],
+                       r.n[b]))
+            T1_off += l.n[ap] * r.n[b]
+    t3 = {}
+    for (ap, w, s1, s2, b) in T1:
+        for a1, a2, v, c in nz.get((w, s1, s2), []):
+            t3.setdefault((ap, a1, a2, v, b), []).append(((ap, w, s1, s2, b), c))
+    T3, T3_off = {}, 0
+    for k in t3:
+        T3[k] = T3_off
+        T3_off += l.n[k[0]] * r.n[k[4]]
+    p.scratch_buf(T1_off)
+    p.scratch_buf(T3_off)
+    s1b, s3b = p.sid(0), p.sid(1)
+    p.gemm(False, False, [(s1b, T1[k], nb, nap, nb, [(xL, lo, na, 0, to, nb, na, 1.0)])
+                          for (k, lo, to, nap, na, nb) in t1])
+    p.lincomb([(s3b, T3[k], l.n[k[0]] * r.n[k[4]], [(s1b, T1[tk], c) for tk, c in tl]) for k, tl in t3.items()])
+    items = []
+    for (ap, a1, a2, bp) in th.keys:
+        sg = []
+        for v in range(len(cR)):
+            b = r.find(r.q[bp] - cR[v])
+            if b is None or (ap, a1, a2, v, b) not in T3 or not R.has((bp, v, b)):
+                continue
+            sg.append((s3b, T3[(ap, a1, a2, v, b)], r.n[b], xR, R.block_off((bp, v, b)), r.n[b], r.n[b], 1.0))
+        items.append((1, th.block_off((ap, a1, a2, bp)), r.n[bp], l.n[ap], r.n[bp], sg))
+    p.gemm(False, True, items)
+    return p.build(ctx, th.size, th.size)
+
+
+def factorize(th: QNTensor, rows_trailing: bool, *, cutoff=0.0, maxdim=None, mindim=1, which_decomp=None, ctx=None):
+    """Truncated split of theta[l, s1, s2, r] over the new link between s1 and s2
+    (inserter.jl:23 with a QN ITensor): sector q = q_l + q_s1 gathers rows (l, s1)
+    and columns (s2, r); ``nsmi_factorize_sectors`` decomposes, truncates globally
+    and keeps per sector; returns (left tensor [l, s1, m], right tensor [m, s2, r],
+    new link, TruncInfo).  rows_trailing puts the isometry on the right tensor."""
+    ctx = ctx or th.ctx
+    l, s1i, s2i, r = th.indices
+    rows, cols = {}, {}
+    for (a, s1, s2, b) in th.keys:
+        q = l.q[a] + s1i.q[s1]
+        rows.setdefault(q, {})[(a, s1)] = None
+        cols.setdefault(q, {})[(s2, b)] = None
+    sectors = sorted(q for q in rows if q in cols)
+    ro, co, R_, C_, M_off = {}, {}, [], [], []
+    tot = 0
+    for q in sectors:
+        off = 0
+        for (a, s1) in sorted(rows[q]):
+            ro[(a, s1)] = off
+            off += l.n[a]
+        R_.append(off)
+        off = 0
+        for (s2, b) in sorted(cols[q]):
+            co[(s2, b)] = off
+            off += r.n[b]
+        C_.append(off)
+        M_off.append(tot)
+        tot += R_[-1] * C_[-1]
+    qi = {q: i for i, q in enumerate(sectors)}
+    Mall = ctx.upload(np.zeros(max(tot, 1)))
+    p = _Program()
+    yM = p.operand(Mall)
+    items = []
+    for (a, s1, s2, b) in th.keys:
+        i = qi[l.q[a] + s1i.q[s1]]
+        items.append((0, th.block_off((a, s1, s2, b)), r.n[b], yM, M_off[i] + ro[(a, s1)] * C_[i] + co[(s2, b)], C_[i],
+                      l.n[a], r.n[b]))
+    p.copy(items)
+    _run_once(p, ctx, th, None)
+    Uall, Call, ks, ti = _factorize_sectors(ctx, Mall, M_off, R_, C_, rows_trailing, cutoff, maxdim, mindim,
+                                            which_decomp)
+    mid = QNIndex([(q, k) for q, k in zip(sectors, ks) if k > 0])
+    left = QNTensor(ctx, [l, s1i, mid], [1, 1, -1], 0)
+    right = QNTensor(ctx, [mid, s2i, r], [1, 1, -1], 0)
+    # offsets of U_q / C_q in the concatenated outputs
+    uoff, coff, uo, cof = {}, {}, 0, 0
+    for i, q in enumerate(sectors):
+        uoff[q], coff[q] = uo, cof
+        k = ks[i]
+        uo += k * (C_[i] if rows_trailing else R_[i])
+        cof += k * (R_[i] if rows_trailing else C_[i])
+    p = _Program()
+    xU, xC = p.operand(Uall), p.operand(Call)
+    yl, yr = p.operand(left.data), p.operand(right.data)
+    items = []
+    for (a, s1, m) in left.keys:
+        q = mid.q[m]
+        i, k = qi[q], mid.n[m]
+        src = xC if rows_trailing else xU
+        base = coff[q] if rows_trailing else uoff[q]
+        items.append((src, base + ro[(a, s1)] * k, k, yl, left.block_off((a, s1, m)), k, l.n[a], k))
+    for (m, s2, b) in right.keys:
+        q = mid.q[m]
+        i, k = qi[q], mid.n[m]
+        src = xU if rows_trailing else xC
+        base = uoff[q] if rows_trailing else coff[q]
+        items.append((src, base + co[(s2, b)], C_[i], yr, right.block_off((m, s2, b)), r.n[b], k, r.n[b]))
+    p.copy(items)
+    op = p.build(ctx, 0, 0)
+    try:
+        op.run(None, None)
+    finally:
+        op.close()
+    return left, right, mid, ti
+
+
+def _factorize_sectors(ctx, Mall, offsets, rows, cols, rows_trailing, cutoff, maxdim, mindim, which_decomp):
+    """nsmi_factorize_sectors: (U_all, C_all, kept per sector, TruncInfo)."""
+    n = len(offsets)
+    route = _lib.ROUTE["auto" if which_decomp is None else which_decomp]
+    t = _lib.Trunc(float(cutoff), int(maxdim) if maxdim is not None else 0, int(mindim), route)
+    kept = (C.c_int64 * max(n, 1))()
+    info = _lib.TruncInfo()
+    Uh, Ch = C.c_void_p(), C.c_void_p()
+    arr = lambda v: (C.c_int64 * max(len(v), 1))(*v)    # noqa: E731
+    ctx.check(ctx.lib.nsmi_factorize_sectors(ctx.h, Mall.h, n, arr(offsets), arr(rows), arr(cols),
+                                             int(bool(rows_trailing)), C.byref(t), C.byref(Uh), C.byref(Ch), kept,
+                                             C.byref(info)))
+    ti = kernels.TruncInfo(info.kept, info.full, info.truncerr, info.total, _lib.ROUTE_NAME.get(info.route, "?"))
+    return DeviceTensor(ctx, Uh), DeviceTensor(ctx, Ch), [int(kept[i]) for i in range(n)], ti
+
+
+# ------------------------------------------------------------ projected operator
+class QNProjTTN:
+    """Environments of a QN chain per directed edge (ProjTTN restricted to a chain),
+    rebuilt when a site on their side changed."""
+
+    def __init__(self, H: QNMPO, ctx: Context):
+        self.H = H
+        self.ctx = ctx
+        self.pos = list(range(len(H)))
+        self.envs = {}
+        self.builds = 0
+        self.env_precision = "fp64"
+
+    def operator(self):
+        return self.H
+
+    def sites(self):
+        return list(self.pos)
+
+    def environment(self, psi: QNMPS, e):
+        u, v = e
+        N = len(self.H)
+        if u < 0:
+            return _boundary_env(self.ctx, psi.links[0], self.H.links[0])
+        if u >= N:
+            return _boundary_env(self.ctx, psi.links[N], self.H.links[N])
+        ent = self.envs.get(e)
+        covered = range(0, u + 1) if v == u + 1 else range(u, N)
+        if ent is not None and all(psi.stamp[x] <= ent[1] for x in covered):
+            return ent[0]
+        if v == u + 1:
+            t = env_left(self.environment(psi, (u - 1, u)), psi[u], self.H.W[u], self.H.c[u], self.H.c[u + 1], self.ctx)
+        else:
+            t = env_right(self.environment(psi, (u + 1, u)), psi[u], self.H.W[u], self.H.c[u], self.H.c[u + 1],
+                          self.ctx)
+        self.builds += 1
+        self.envs[e] = (t, psi.clock)
+        return t
+
+    def position(self, psi: QNMPS, region):
+        self.pos = sorted(region)
+        lo, hi = self.pos[0], self.pos[-1]
+        self.environment(psi, (lo - 1, lo))
+        self.environment(psi, (hi + 1, hi))
+        return self
+
+    def left_right(self, psi):
+        lo, hi = min(self.pos), max(self.pos)
+        return self.environment(psi, (lo - 1, lo)), self.environment(psi, (hi + 1, hi))
+
+
+# ------------------------------------------------------------ region step hooks
+def extract(psi: QNMPS, region):
+    lo, hi = min(region), max(region)
+    if len(region) != 2 or hi != lo + 1:
+        raise NotImplementedError("QN chains: 2-site regions on neighbouring sites are served")
+    if psi.ortho_region is None or not all(lo <= v <= hi for v in psi.ortho_region):
+        raise NotImplementedError("QN chains: the orthogonality centre must already lie in the region "
+                                  "(sweeps from product states keep it there)")
+    psi.ortho_region = list(region)
+    return theta(psi[lo], psi[hi], psi.ctx)
+
+
+def heff_map(P: QNProjTTN, psi: QNMPS):
+    lo, hi = min(P.pos), max(P.pos)
+    L, R = P.left_right(psi)
+    th = QNTensor(psi.ctx, [psi.links[lo], psi.site, psi.site, psi.links[hi + 1]], [1, 1, 1, -1], 0)
+    return heff2(L, P.H.W[lo], P.H.W[hi], R, th, P.H.c[lo], P.H.c[hi + 1], psi.ctx)
+
+
+def insert(psi: QNMPS, region, th: QNTensor, trunc, which_decomp=None):
+    u, v = region
+    lo, hi = min(u, v), max(u, v)
+    left, right, mid, info = factorize(th, u != lo, cutoff=trunc["cutoff"], maxdim=trunc["maxdim"],
+                                       mindim=trunc["mindim"], which_decomp=which_decomp, ctx=psi.ctx)
+    psi.links[lo + 1] = mid
+    psi[lo] = left
+    psi[hi] = right
+    return info
+
+
+def expect_mpo(psi: QNMPS, H: QNMPO) -> float:
+    """<psi|H|psi> / <psi|psi> by a left-to-right environment pass (and one with the
+    identity operator for the norm)."""
+    ctx = psi.ctx
+    E = _boundary_env(ctx, psi.links[0], H.links[0])
+    for i in range(len(psi)):
+        E = env_left(E, psi[i], H.W[i], H.c[i], H.c[i + 1], ctx)
+    d = psi.site.dim
+    one = np.eye(d).reshape(1, 1, d, d)
+    z = np.zeros(1, dtype=np.int64)
+    N1 = _boundary_env(ctx, psi.links[0], QNIndex([(0, 1)]))
+    for i in range(len(psi)):
+        N1 = env_left(N1, psi[i], one, z, z, ctx)
+    return float(E.data.numpy().sum()) / float(N1.data.numpy().sum())
+
+
+__all__ = ["QNIndex", "QNTensor", "QNMPO", "QNMPS", "QNProjTTN", "spin_site_index", "qn_product_mps", "theta",
+           "env_left", "env_right", "heff2", "factorize", "expect_mpo", "BlockProgram"]

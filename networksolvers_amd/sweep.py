@@ -24,7 +24,7 @@ import functools
 import itertools
 from dataclasses import dataclass, field
 
-from . import kernels, subspace, tree
+from . import kernels, qn, subspace, tree
 from .network import ProjTTN, TTN, TTNO, orthogonalize
 from .region_plans import euler_sweep, truncation_parameters
 
@@ -193,6 +193,14 @@ def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=Non
         raise NotImplementedError(
             "Subspace expansion (subspace_expand!) not defined for requested combination of "
             f"subspace_algorithm={subspace_algorithm!r} and problem types")
+    if isinstance(state(problem), qn.QNMPS):
+        if subspace_algorithm is not None:
+            raise NotImplementedError("Subspace expansion (subspace_expand!) not defined for requested combination "
+                                      "of subspace_algorithm and problem types (QN chains)")
+        psi = state(problem)
+        local_state = qn.extract(psi, list(region))
+        problem.operator = operator(problem).position(psi, list(region))
+        return problem, local_state
     if isinstance(state(problem), tree.TreeTTN):
         psi = state(problem)
         local_state = tree.extract(psi, list(region))
@@ -221,6 +229,10 @@ def optimal_map(P: ProjTTN, psi: TTN, precision="fp64") -> kernels.HeffMap:
     """operator_map.jl:3-10: the region's effective Hamiltonian as a device map
     (environments of the incident edges and the region's MPO tensors; on a tree a
     contraction program over up to four environments)."""
+    if isinstance(P, qn.QNProjTTN):
+        if precision != "fp64":
+            raise ValueError("QN regions are served in FP64 only")
+        return qn.heff_map(P, psi)
     if isinstance(P, tree.TreeProjTTN):
         if precision != "fp64":
             raise ValueError("tree regions are served in FP64 (ComplexF64) only")
@@ -273,7 +285,11 @@ def inserter(problem, local_tensor, region_iter, *, normalize=False, set_orthogo
     region = current_region(region_iter)
     psi = state(problem)
     info = None
-    if isinstance(psi, tree.TreeTTN):
+    if isinstance(psi, qn.QNMPS):
+        if len(region) != 2:
+            raise NotImplementedError("QN chains: 2-site regions are served")
+        info = qn.insert(psi, list(region), local_tensor, t, which_decomp)
+    elif isinstance(psi, tree.TreeTTN):
         if len(region) > 2:
             raise ValueError(f"Region of length {len(region)} not currently supported")
         info = tree.insert(psi, list(region), local_tensor, t, which_decomp)
@@ -331,6 +347,8 @@ def projected_operator(H, env_precision="fp64"):
     """ProjTTN of a chain MPO or a tree TTN-MPO (eigsolve.jl:71, applyexp.jl:86)."""
     if isinstance(H, tree.TreeTTNO):
         return tree.TreeProjTTN(H, env_precision)
+    if isinstance(H, qn.QNMPO):
+        return qn.QNProjTTN(H, H.ctx)
     return ProjTTN(H, env_precision)
 
 

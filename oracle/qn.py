@@ -1,0 +1,218 @@
+"""U(1) quantum-number DMRG on a chain (test infrastructure only).
+
+Restates the reference's 2-site sweep on QN-conserving tensors (README.md QN
+timings; ITensors QN ITensors under eigsolve.jl / inserter.jl) with DENSE numpy
+tensors whose charge-forbidden blocks are zero: environments, the effective
+Hamiltonian and the Lanczos solver are the dense chain oracle's (the zero blocks
+stay exactly zero), and only the truncated split is block-wise -- NDTensors'
+block-sparse svd/eigen: each charge sector of theta (rows (l, s1) and columns
+(s2, r) with q_l + q_s1 = q_r - q_s2 = q) is decomposed on its own, the sector
+spectra are merged and truncated together by truncate!, and each sector keeps
+its entries above the cut value docut (so a near-degenerate pair at the cut is
+kept or dropped together).  The new link lists its sectors in increasing charge,
+each with its kept vectors in decreasing spectral order.
+
+Pinned by (tests/test_oracle_qn.py): without truncation the sector split
+reproduces theta exactly and its merged spectrum equals the dense one; the QN
+ground energy of the spin-1 chain N=10 equals the reference's constant
+-12.8945601 and the ED fixture; QN and dense DMRG energies agree.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import linalg
+from .chain import ChainEnvs, heff2_apply, theta2
+from .plans import euler_sweep_chain, get_or_last
+
+
+def spin_charges(d):
+    """2m for m = S, S-1, ..., -S."""
+    return np.array([int(round(2 * ((d - 1) / 2 - i))) for i in range(d)])
+
+
+def mpo_link_charges(W, qs):
+    """MPO link charges from the left boundary: c_wr = c_wl + q_s' - q_s."""
+    c = [np.zeros(W[0].shape[0], dtype=int)]
+    for w in W:
+        cr = {}
+        for a, b, sp, s in zip(*np.nonzero(w)):
+            v = c[-1][a] + qs[sp] - qs[s]
+            if cr.setdefault(b, v) != v:
+                raise ValueError("MPO does not conserve the charge")
+        c.append(np.array([cr.get(b, 0) for b in range(w.shape[1])]))
+    return c
+
+
+@dataclass
+class Link:
+    """Sectors (charge, dim) in increasing charge; dense basis = sectors in order."""
+    sectors: list
+
+    def charges(self):
+        return np.concatenate([np.full(n, q) for q, n in self.sectors]) if self.sectors else np.zeros(0, int)
+
+    def offsets(self):
+        out, o = {}, 0
+        for q, n in self.sectors:
+            out[q] = o
+            o += n
+        return out
+
+
+def _sector_decomp(M, route):
+    """(P sorted as NDTensors sorts it, row vectors V (len(P) x m) of the isometry
+    over M's rows, in that order)."""
+    m, n = M.shape
+    if route == "svd":
+        u, s, vh = np.linalg.svd(M, full_matrices=False)
+        return s.astype(float) ** 2, u.T
+    w, v = np.linalg.eigh(M @ M.T)
+    order = np.argsort(-np.abs(w), kind="stable")
+    return w[order], v[:, order].T
+
+
+def factorize_sectors(mats, rows_trailing, *, cutoff=0.0, maxdim=None, mindim=1, which_decomp=None):
+    """Block-diagonal truncated split of the sector matrices ``mats`` (list of
+    2-D arrays).  Returns (list of (U_q, C_q) or None per sector, kept list,
+    truncerr): U_q is rows x k (k x cols when rows_trailing) with the largest-
+    magnitude entry of each isometry vector positive; C_q = U_q^T M (M U_q^T)."""
+    route = linalg.choose_route(cutoff, which_decomp)
+    dec = []
+    for M in mats:
+        Mi = M.T if rows_trailing else M
+        dec.append(_sector_decomp(Mi, route))
+    full = sum(min(M.shape) for M in mats)
+    mx = full if maxdim is None else min(int(maxdim), full)
+    key = (lambda v: abs(v)) if route == "eigen" else (lambda v: v)
+    ent = [(key(p), p, q, i) for q, (P, _) in enumerate(dec) for i, p in enumerate(P)]
+    ent.sort(key=lambda e: -e[0])           # stable: sector order, then spectral order, on ties
+    P = np.array([e[1] for e in ent], dtype=float)
+    n, truncerr, docut = linalg.truncate(P, mindim=mindim, maxdim=mx, cutoff=cutoff)
+    Pp = P.copy()                           # the values truncate! compares (sign flip, negative tail zeroed)
+    if len(Pp) and Pp[0] < 0:
+        Pp = -Pp
+    for j in range(len(Pp) - 1, -1, -1):
+        if Pp[j] >= 0:
+            break
+        Pp[j] = 0.0
+    kept = [0] * len(mats)
+    if n < len(P):
+        for j, e in enumerate(ent):
+            if Pp[j] > docut:
+                kept[e[2]] += 1
+    else:
+        kept = [len(d[0]) for d in dec]
+    if sum(kept) == 0 and ent:
+        kept[ent[0][2]] = 1
+    out = []
+    for q, M in enumerate(mats):
+        k = kept[q]
+        if k == 0:
+            out.append(None)
+            continue
+        V = dec[q][1][:k].T                 # iso x k
+        V, _ = linalg.fix_column_phases(V)
+        if not rows_trailing:
+            out.append((V, V.T @ M))
+        else:
+            out.append((V.T, M @ V))
+    return out, kept, truncerr
+
+
+def qn_split(theta, ql, qs, qr_link, rows_trailing, **trunc):
+    """The split of a dense theta[l, s1, s2, r] by charge sectors: returns the
+    dense left tensor [l, s1, m], right tensor [m, s2, r], the new link, truncerr."""
+    cl, d, _, cr = theta.shape
+    lo = ql.offsets()
+    ro = qr_link.offsets()
+    rows, cols = {}, {}
+    for (qa, na) in ql.sectors:
+        for s1 in range(d):
+            rows.setdefault(qa + qs[s1], []).append((qa, s1, na))
+    for (qb, nb) in qr_link.sectors:
+        for s2 in range(d):
+            cols.setdefault(qb - qs[s2], []).append((s2, qb, nb))
+    sectors = sorted(q for q in rows if q in cols)
+    mats, ridx, cidx = [], [], []
+    for q in sectors:
+        # rows sorted by (sector of l, s1), then the index inside the sector; columns by (s2, sector of r)
+        ri = []
+        for (qa, s1, na) in sorted(rows[q], key=lambda t: (ql.sectors.index((t[0], t[2])), t[1])):
+            ri += [(lo[qa] + i, s1) for i in range(na)]
+        ci = []
+        for (s2, qb, nb) in sorted(cols[q], key=lambda t: (t[0], qr_link.sectors.index((t[1], t[2])))):
+            ci += [(s2, ro[qb] + j) for j in range(nb)]
+        ra, rs = np.array([x[0] for x in ri]), np.array([x[1] for x in ri])
+        cs, cb = np.array([x[0] for x in ci]), np.array([x[1] for x in ci])
+        M = theta[ra[:, None], rs[:, None], cs[None, :], cb[None, :]]
+        mats.append(M)
+        ridx.append(ri)
+        cidx.append(ci)
+    res, kept, truncerr = factorize_sectors(mats, rows_trailing, **trunc)
+    mid = Link([(int(q), int(k)) for q, k in zip(sectors, kept) if k > 0])
+    K = sum(k for _, k in mid.sectors)
+    left = np.zeros((cl, d, K))
+    right = np.zeros((K, d, cr))
+    mo = mid.offsets()
+    for q, r_, ri, ci in zip(sectors, res, ridx, cidx):
+        if r_ is None:
+            continue
+        U, Cq = r_
+        Lq, Rq = (Cq, U) if rows_trailing else (U, Cq)      # left: rows x k, right: k x cols
+        k = Lq.shape[1]
+        ra, rs = np.array([x[0] for x in ri]), np.array([x[1] for x in ri])
+        cs, cb = np.array([x[0] for x in ci]), np.array([x[1] for x in ci])
+        left[ra, rs, mo[q]:mo[q] + k] = Lq
+        right[mo[q]:mo[q] + k, cs, cb] = Rq
+    return left, right, mid, truncerr
+
+
+@dataclass
+class QNResult:
+    energy: float
+    mps: list
+    links: list
+    log: list = field(default_factory=list)
+
+
+def qn_dmrg_chain(W, states, d, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, krylovdim=3, tol=1e-14,
+                  which_decomp=None):
+    """2-site QN DMRG from a product state (basis state states[i] on site i) over
+    the Euler-tour plan from the last site; log rows (region, eigenvalue, kept,
+    truncerr, new link sectors)."""
+    N = len(W)
+    qs = spin_charges(d)
+    links = [Link([(0, 1)])]
+    q = 0
+    mps = []
+    for s in states:
+        q += qs[s]
+        links.append(Link([(int(q), 1)]))
+        a = np.zeros((1, d, 1))
+        a[0, s, 0] = 1.0
+        mps.append(a)
+    envs = ChainEnvs(W)
+    log, E = [], None
+    for sweep in range(1, nsweeps + 1):
+        cut = get_or_last(cutoff, sweep)
+        mxd = get_or_last(maxdim, sweep) if maxdim is not None else None
+        mnd = get_or_last(mindim, sweep)
+        for (u, v) in euler_sweep_chain(N, N - 1):
+            lo, hi = min(u, v), max(u, v)
+            L = envs.get(mps, lo - 1, lo)
+            R = envs.get(mps, hi + 1, hi)
+            th = theta2(mps[lo], mps[hi])
+            shape = th.shape
+            E, x, _ = linalg.lanczos_eigsolve(lambda y: heff2_apply(L, W[lo], W[hi], R, y.reshape(shape)).reshape(-1),
+                                              th.reshape(-1), krylovdim=krylovdim, tol=tol)
+            left, right, mid, te = qn_split(x.reshape(shape), links[lo], qs, links[hi + 1], u != lo, cutoff=cut,
+                                            maxdim=mxd, mindim=mnd, which_decomp=which_decomp)
+            mps[lo], mps[hi] = left, right
+            links[lo + 1] = mid
+            envs.site_changed(lo)
+            envs.site_changed(hi)
+            log.append(((u, v), E, sum(k for _, k in mid.sectors), te, list(mid.sectors)))
+    return QNResult(E, mps, links, log)

@@ -1,0 +1,162 @@
+"""GPU parity of the U(1) block-sparse path (block programs, sector factorize, QN
+DMRG) against dense contractions and the oracle.
+
+Bars: every block-program contraction equals the dense contraction of the same
+tensors to 1e-12 relative; the sector split equals the oracle's (same new link
+sectors, tensors element-wise to 1e-10 on non-degenerate random spectra); QN DMRG
+equals the oracle (energy 1e-10 relative, identical link sectors and kept counts,
+1 - fidelity < 1e-10) and the dense device DMRG (energy 1e-10, bond dimensions).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import chain, models
+from oracle import qn as oqn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import networksolvers_amd as ns
+    c = ns.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _ed(model, key):
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")) as f:
+        return json.load(f)[model][key]
+
+
+def _rand_qn(ctx, indices, dirs, seed):
+    from networksolvers_amd import qn
+    t = qn.QNTensor(ctx, indices, dirs, 0)
+    t.data = ctx.upload(np.random.default_rng(seed).standard_normal(max(t.size, 1)))
+    return t
+
+
+def _setup(ctx):
+    from networksolvers_amd import qn
+    site = qn.spin_site_index(2)
+    l = qn.QNIndex([(-2, 3), (0, 4), (2, 2)])
+    m = qn.QNIndex([(-3, 2), (-1, 5), (1, 6), (3, 2)])
+    r = qn.QNIndex([(-4, 1), (-2, 4), (0, 5), (2, 3), (4, 2)])
+    H = qn.QNMPO(ctx, models.heisenberg_mpo(4), site)
+    return site, l, m, r, H
+
+
+def test_qn_theta_and_environments(ctx):
+    from networksolvers_amd import qn
+    site, l, m, r, H = _setup(ctx)
+    A = _rand_qn(ctx, [l, site, m], [1, 1, -1], 1)
+    B = _rand_qn(ctx, [m, site, r], [1, 1, -1], 2)
+    th = qn.theta(A, B, ctx)
+    assert rel(th.to_dense(), chain.theta2(A.to_dense(), B.to_dense())) < 1e-13
+    W, cl, cr = H.W[1], H.c[1], H.c[2]
+    EL = _rand_qn(ctx, [l, H.links[1], l], [1, -1, -1], 3)
+    got = qn.env_left(EL, A, W, cl, cr, ctx).to_dense()
+    assert rel(got, chain.env_left(EL.to_dense(), A.to_dense(), W)) < 1e-12
+    ER = _rand_qn(ctx, [m, H.links[2], m], [1, -1, -1], 4)
+    got = qn.env_right(ER, A, W, cl, cr, ctx).to_dense()
+    assert rel(got, chain.env_right(ER.to_dense(), A.to_dense(), W)) < 1e-12
+
+
+def test_qn_heff2_apply(ctx):
+    from networksolvers_amd import qn
+    site, l, m, r, H = _setup(ctx)
+    L = _rand_qn(ctx, [l, H.links[1], l], [1, -1, -1], 5)
+    R = _rand_qn(ctx, [r, H.links[3], r], [1, -1, -1], 6)
+    th = _rand_qn(ctx, [l, site, site, r], [1, 1, 1, -1], 7)
+    op = qn.heff2(L, H.W[1], H.W[2], R, th, H.c[1], H.c[3], ctx)
+    y = qn.QNTensor(ctx, th.indices, th.dirs, 0, th.keys, op(th.data))
+    want = chain.heff2_apply(L.to_dense(), H.W[1], H.W[2], R.to_dense(), th.to_dense())
+    assert rel(y.to_dense(), want) < 1e-12
+    assert abs(op.rayleigh(th.data) - np.vdot(th.to_dense(), want) / np.vdot(th.to_dense(), th.to_dense())) < 1e-10
+    op.close()
+
+
+@pytest.mark.parametrize("rows_trailing", [False, True])
+@pytest.mark.parametrize("cutoff,maxdim", [(1e-14, None), (1e-6, 8), (1e-3, None)])
+def test_qn_factorize_vs_oracle(ctx, rows_trailing, cutoff, maxdim):
+    from networksolvers_amd import qn
+    site, l, m, r, H = _setup(ctx)
+    th = _rand_qn(ctx, [l, site, site, r], [1, 1, 1, -1], 8)
+    left, right, mid, info = qn.factorize(th, rows_trailing, cutoff=cutoff, maxdim=maxdim, ctx=ctx)
+    ol = oqn.Link([(q, n) for q, n in zip(l.q, l.n)])
+    orr = oqn.Link([(q, n) for q, n in zip(r.q, r.n)])
+    Lo, Ro, mo, te = oqn.qn_split(th.to_dense(), ol, oqn.spin_charges(2), orr, rows_trailing, cutoff=cutoff,
+                                  maxdim=maxdim)
+    assert list(zip(mid.q, mid.n)) == [tuple(x) for x in mo.sectors]
+    assert rel(left.to_dense(), Lo) < 1e-10
+    assert rel(right.to_dense(), Ro) < 1e-10
+    assert abs(info.truncerr - te) < 1e-12 + 1e-8 * te
+
+
+def test_qn_dmrg_spin1_vs_oracle(ctx):
+    """examples/dmrg.jl with conserve_qns: S=1 chain N=10, Neel start, Sz=0 sector."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import qn
+    N = 10
+    W = models.heisenberg_mpo(N, d=3)
+    states = [0 if (i + 1) % 2 == 0 else 2 for i in range(N)]
+    site = qn.spin_site_index(3)
+    H = qn.QNMPO(ctx, W, site)
+    prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site), operator=ns.sweep.projected_operator(H))
+    trunc = dict(cutoff=1e-12, maxdim=[10, 40, 80, 160, 243])
+    E, psi = ns.sweep.eigsolve_problem(prob, nsweeps=6, nsites=2, inserter_kwargs=dict(trunc=trunc))
+    ref = oqn.qn_dmrg_chain(W, states, 3, nsweeps=6, maxdim=[10, 40, 80, 160, 243], cutoff=1e-12)
+    assert abs(E - (-12.8945601)) < 1e-7
+    assert abs(E - ref.energy) < 1e-10 * abs(E)
+    assert [r_[2] for r_ in prob.region_log] == [r_[2] for r_ in ref.log]
+    for i in range(N + 1):
+        assert list(zip(psi.links[i].q, psi.links[i].n)) == [tuple(x) for x in ref.links[i].sectors]
+    a = chain.mps_to_dense(psi.to_arrays()).reshape(-1)
+    b = chain.mps_to_dense(ref.mps).reshape(-1)
+    assert 1 - abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a) * np.vdot(b, b)).real < 1e-10
+    assert abs(ns.expect_mpo(psi, H) - E) < 1e-9
+
+
+def test_qn_equals_dense_device_dmrg(ctx):
+    import networksolvers_amd as ns
+    from networksolvers_amd import qn
+    N = 16
+    W = models.heisenberg_mpo(N)
+    states = [0, 1] * (N // 2)
+    site = qn.spin_site_index(2)
+    trunc = dict(cutoff=1e-10, maxdim=[8, 16, 32])
+    Eq, pq = ns.sweep.eigsolve_problem(
+        ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site),
+                           operator=ns.sweep.projected_operator(qn.QNMPO(ctx, W, site))),
+        nsweeps=5, nsites=2, inserter_kwargs=dict(trunc=trunc))
+    Ed, pd = ns.dmrg(ns.TTNO(ctx, W), ns.TTN.from_arrays(ctx, chain.product_mps(states, 2)), nsweeps=5, nsites=2,
+                     inserter_kwargs=dict(trunc=trunc))
+    assert abs(Eq - Ed) < 1e-10 * abs(Ed)
+    assert pq.linkdims() == pd.linkdims()
+    assert abs(Eq - _ed("heisenberg_s12_chain", "16")) < 1e-6
+
+
+def test_qn_c3_spin1_n64_sweeps_vs_oracle(ctx):
+    """The block-sparse configuration's chain (spin-1 Heisenberg N=64, Sz=0, Neel
+    start) for 3 sweeps at maxdim 16, 32, 64 against the oracle."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import qn
+    N = 64
+    W = models.heisenberg_mpo(N, d=3)
+    states = [0 if (i + 1) % 2 == 0 else 2 for i in range(N)]
+    site = qn.spin_site_index(3)
+    prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site),
+                              operator=ns.sweep.projected_operator(qn.QNMPO(ctx, W, site)))
+    E, psi = ns.sweep.eigsolve_problem(prob, nsweeps=3, nsites=2,
+                                       inserter_kwargs=dict(trunc=dict(cutoff=1e-10, maxdim=[16, 32, 64])))
+    ref = oqn.qn_dmrg_chain(W, states, 3, nsweeps=3, maxdim=[16, 32, 64], cutoff=1e-10)
+    assert abs(E - ref.energy) < 1e-9 * abs(E)
+    for i in range(N + 1):
+        assert list(zip(psi.links[i].q, psi.links[i].n)) == [tuple(x) for x in ref.links[i].sectors]
