@@ -1,0 +1,341 @@
+// Symmetric tridiagonalisation of the truncation's Gram matrix rho (n x n) in one
+// cooperative launch (the eigensolver behind inserter.jl:23 -> ITensors
+// eigen/svd -> LAPACK syevd).  rocSOLVER's blocked sytrd issues about ten small
+// kernels per column and is launch-latency bound at the sizes a sweep produces
+// (about 17 us per column on gfx950 for n <= 2048); here a column costs one grid
+// barrier and a few microseconds of register-resident work.
+//
+// Householder reduction with LAPACK dsytd2 "lower" semantics on the column-major
+// view (rho is symmetric, so the row-major buffer is the same matrix):
+//   for j = 0 .. n-2:  x = A(j+1:n, j);  alpha = x_0, sigma^2 = |x_1:|^2
+//     sigma^2 == 0:    tau_j = 0, e_j = alpha
+//     else:            beta = -sign(alpha) sqrt(alpha^2 + sigma^2), tau_j = (beta - alpha)/beta,
+//                      v = [1, x_1:/(alpha - beta)], e_j = beta
+//     p = tau_j A22 v,  w = p - (tau_j/2)(p.v) v,  A22 -= v w^T + w v^T,  d_j = A(j, j)
+// The output has LAPACK's layout -- d, e, tau, and in the lower triangle of the
+// column-major view the diagonal d, the subdiagonal e and v_j(2:) in A(j+2:n, j)
+// -- which rocsolver_dstedc and rocsolver_dormtr consume.
+//
+// Distribution: workgroup g owns the rows i = g (mod G); row i sits in the
+// registers of one wave (slot s of wave w holds row (4s + w) G + g), lane l
+// holding the columns l + 64c.  Step j:
+//   1. every workgroup forms v from the column x_j (kept in LDS by all of them);
+//   2. each wave computes p_i = tau v.row_i for its rows i > j and publishes it;
+//      the owner of row j+1 publishes that row (= column j+1, pre-update) as Y;
+//   3. grid barrier;
+//   4. every workgroup reads p and Y, forms w, its own copy of the next column
+//      x_{j+1} = Y - v w_{j+1} - w v_{j+1} and d_{j+1} = Y_{j+1} - 2 w_{j+1},
+//      and applies the rank-2 update to the rows it owns.
+// so one barrier per column suffices.  The exchange buffers alternate by the
+// parity of j: a workgroup writes step j+1's values only after passing barrier j
+// and before barrier j+1, when every workgroup has finished reading step j-1's.
+// Barrier spins are bounded: a workgroup that waits too long raises a flag and
+// leaves, every other one leaves at its next barrier once it sees the flag, so
+// the grid always drains; the host then reports the failure.
+#include "internal.h"
+
+namespace nsmi {
+namespace {
+
+constexpr int kTriThreads = 256;
+
+// DPP lane move of a double (two 32-bit moves); lanes disabled by the row mask or
+// reading outside the row take 0
+template <int CTRL, int ROWM>
+__device__ inline double dpp_mov(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWM, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWM, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double read_lane63(double x) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 63),
+                          __builtin_amdgcn_readlane(__double2loint(x), 63));
+}
+
+// Wave-wide sums of R values at once (the R chains interleave); every lane gets
+// the totals.  Pair, quad, half-row and row sums by quad permutes and mirrors,
+// then rows 1 and 3 take row 0 and 2 (row_bcast:15) and rows 2 and 3 take the
+// sum of rows 0 and 1 (row_bcast:31): lane 63 holds the total.
+template <int R>
+__device__ inline void wave_totals(double (&a)[R]) {
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0xB1, 0xF>(a[s]);  // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x4E, 0xF>(a[s]);  // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x141, 0xF>(a[s]);  // row_half_mirror
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x140, 0xF>(a[s]);  // row_mirror
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x142, 0xA>(a[s]);  // row_bcast:15
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x143, 0xC>(a[s]);  // row_bcast:31
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] = read_lane63(a[s]);
+}
+
+__device__ inline double wave_total(double x) {
+  double a[1] = {x};
+  wave_totals<1>(a);
+  return a[0];
+}
+
+// orders a wave's own LDS writes before its later LDS reads of other lanes' entries
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Grid-wide barrier on a monotone arrival counter (target = G * number of the
+// barrier).  Thread 0 publishes the workgroup's writes (agent-scope fence),
+// arrives, and spins with an acquire load; the spin gives up after a bounded
+// number of polls or when another workgroup has raised the failure flag.
+__device__ inline bool grid_barrier(unsigned* bar, unsigned target, unsigned* fail, int* s_ok) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(bar, 1u);
+    int ok = 1;
+    for (unsigned spins = 0; __hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target; ++spins) {
+      __builtin_amdgcn_s_sleep(1);
+      if ((spins & 255u) == 255u && __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = 0;
+        break;
+      }
+      if (spins > (1u << 22)) {
+        atomicOr(fail, 1u);
+        ok = 0;
+        break;
+      }
+    }
+    __threadfence();
+    *s_ok = ok;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// CL: 64-column slots per lane (n <= 64 CL); RW: rows per wave (G = ceil(n / (4 RW))).
+// SINGLE: one workgroup (G == 1), the p / Y exchange in LDS and the barrier a
+// workgroup barrier.  xch: 4 n doubles of global exchange (P[2][n], Y[2][n]) when
+// G > 1; bar, fail: zero on entry.
+//
+// Every wave keeps its own register copy of x_j, v and w at its lanes' columns
+// (they are cheap to form redundantly), and a wave-private LDS copy of v and w
+// for the broadcast reads v_i, w_i of its rows; so the only synchronisation of a
+// step is the barrier between publishing p / Y and reading them.
+template <int CL, int RW, bool SINGLE>
+__global__ __launch_bounds__(kTriThreads) void k_sytrd_coop(double* __restrict__ A, int n, double* __restrict__ d,
+                                                            double* __restrict__ e, double* __restrict__ tau,
+                                                            double* __restrict__ xch, unsigned* bar,
+                                                            unsigned* fail) {
+  constexpr int NP = 64 * CL;  // padded vector length
+  __shared__ double vl[4][NP], wl[4][NP];
+  __shared__ double xl[SINGLE ? 4 * NP : 1];
+  __shared__ int s_ok;
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double* __restrict__ vme = vl[wave];
+  double* __restrict__ wme = wl[wave];
+  double* const xbase = SINGLE ? xl : xch;
+  const int xn = SINGLE ? NP : n;  // stride of the exchange halves
+
+  double row[RW][CL];
+  int rid[RW];
+#pragma unroll
+  for (int s = 0; s < RW; ++s) {
+    const int i = (s * 4 + wave) * G + g;
+    rid[s] = i < n ? i : -1;
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      const int k = lane + 64 * c;
+      row[s][c] = (i < n && k < n) ? A[size_t(i) * n + k] : 0.0;
+    }
+  }
+  // x_0 = column 0 = row 0 (symmetric); nobody writes A before the first barrier
+  double xs[CL];
+#pragma unroll
+  for (int c = 0; c < CL; ++c) {
+    const int k = lane + 64 * c;
+    xs[c] = (k >= 1 && k < n) ? A[k] : 0.0;
+  }
+  double alpha = n > 1 ? A[1] : 0.0;
+  if (g == 0 && tid == 0) d[0] = A[0];
+
+  unsigned nbar = 0;
+  for (int j = 0; j + 1 < n; ++j) {
+    double* P = xbase + (j & 1) * xn;
+    double* Y = xbase + 2 * xn + (j & 1) * xn;
+    // 1. reflector from x = x_j[j+1 ..]
+    double part = 0.0;
+#pragma unroll
+    for (int c = 0; c < CL; ++c)
+      if (lane + 64 * c >= j + 2) part += xs[c] * xs[c];
+    const double sigma2 = wave_total(part);
+    double t = 0.0, beta = alpha, scal = 0.0;
+    if (sigma2 != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma2), alpha);
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    double vv[CL];
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      const int k = lane + 64 * c;
+      vv[c] = (k == j + 1) ? 1.0 : (k >= j + 2 ? xs[c] * scal : 0.0);
+      vme[k] = vv[c];
+    }
+    // 2. p_i = tau v . row_i for own rows i > j; the owner of row j+1 publishes it as Y
+    double acc[RW];
+#pragma unroll
+    for (int s = 0; s < RW; ++s) {
+      acc[s] = 0.0;
+#pragma unroll
+      for (int c = 0; c < CL; ++c)
+        if (64 * c + 63 > j) acc[s] += row[s][c] * vv[c];
+    }
+    wave_totals<RW>(acc);
+#pragma unroll
+    for (int s = 0; s < RW; ++s) {
+      if (rid[s] <= j) continue;  // uniform per wave (rid < 0: empty slot)
+      if (lane == 0) P[rid[s]] = t * acc[s];
+      if (rid[s] == j + 1) {
+#pragma unroll
+        for (int c = 0; c < CL; ++c) {
+          const int k = lane + 64 * c;
+          if (k > j && k < n) Y[k] = row[s][c];
+        }
+      }
+    }
+    // 3.
+    if (SINGLE) {
+      __syncthreads();
+    } else if (!grid_barrier(bar, unsigned(G) * ++nbar, fail, &s_ok)) {
+      return;
+    }
+    if (g == 0 && wave == 0) {
+      // LAPACK layout of column j (column-major view = row j of the buffer)
+#pragma unroll
+      for (int c = 0; c < CL; ++c) {
+        const int k = lane + 64 * c;
+        if (k >= j + 2 && k < n) A[size_t(j) * n + k] = vv[c];
+      }
+      if (lane == 0) {
+        e[j] = beta;
+        tau[j] = t;
+        A[size_t(j) * n + j + 1] = beta;
+      }
+    }
+    // 4. w = p - (tau/2)(p.v) v, the next column and d_{j+1}
+    double ww[CL];
+    part = 0.0;
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      const int k = lane + 64 * c;
+      ww[c] = (k > j && k < n) ? P[k] : 0.0;
+      part += ww[c] * vv[c];
+    }
+    const double cc = 0.5 * t * wave_total(part);
+    const double w1 = P[j + 1] - cc;  // w_{j+1}, as v_{j+1} = 1
+#pragma unroll
+    for (int c = 0; c < CL; ++c) {
+      const int k = lane + 64 * c;
+      ww[c] -= cc * vv[c];
+      wme[k] = ww[c];
+      xs[c] = (k >= j + 2 && k < n) ? Y[k] - vv[c] * w1 - ww[c] : 0.0;
+    }
+    if (g == 0 && tid == 0) {
+      const double dn = Y[j + 1] - 2.0 * w1;
+      d[j + 1] = dn;
+      A[size_t(j + 1) * n + j + 1] = dn;
+    }
+    wave_sync_lds();
+    alpha = (j + 2 < n) ? Y[j + 2] - vme[j + 2] * w1 - wme[j + 2] : 0.0;
+#pragma unroll
+    for (int s = 0; s < RW; ++s) {
+      if (rid[s] <= j) continue;
+      const double vi = vme[rid[s]], wi = wme[rid[s]];
+#pragma unroll
+      for (int c = 0; c < CL; ++c)
+        if (64 * c + 63 > j) row[s][c] -= vi * ww[c] + wi * vv[c];
+    }
+  }
+}
+
+struct TriPlan {
+  const void* fn;
+  int rows_per_group;
+};
+
+// the smallest instantiation whose padded width covers n
+TriPlan tri_plan(int n) {
+  if (n <= 64) return {reinterpret_cast<const void*>(k_sytrd_coop<1, 16, true>), 64};
+  if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_coop<2, 32, true>), 128};
+  if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_coop<4, 16, false>), 64};
+  if (n <= 512) return {reinterpret_cast<const void*>(k_sytrd_coop<8, 8, false>), 32};
+  return {reinterpret_cast<const void*>(k_sytrd_coop<16, 4, false>), 16};
+}
+
+}  // namespace
+
+int sytrd_coop_groups(int n) {
+  if (n < 2 || n > kSytrdCoopMax) return 0;
+  const TriPlan p = tri_plan(n);
+  return (n + p.rows_per_group - 1) / p.rows_per_group;
+}
+
+// A: n x n symmetric on the device (row-major == column-major), overwritten with
+// LAPACK's lower-triangle tridiagonal layout; d (n), e and tau (n - 1) on the
+// device.  Returns false, with nothing launched, when n is outside the supported
+// range or the device cannot co-schedule the grid (the caller then uses the
+// library path).
+//
+// Grids of up to kPlainLaunchMax workgroups go out as ordinary launches: a
+// cooperative launch waits for the whole device to drain, which would serialise
+// the independent solves that share the GPU, and so few workgroups (one per CU
+// of 256) are co-resident in practice even beside other concurrent reductions;
+// the bounded barrier spin turns the pathological case into an error, not a
+// hang.  Larger grids use the cooperative launch and its co-residency check.
+bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau) {
+  constexpr int kPlainLaunchMax = 16;
+  const int G = sytrd_coop_groups(n);
+  if (G == 0) return false;
+  const TriPlan p = tri_plan(n);
+  if (G > kPlainLaunchMax) {
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    NSMI_HIP(hipGetDevice(&dev));
+    NSMI_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    NSMI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    NSMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, kTriThreads, 0));
+    if (!coop || per_cu < 1 || G > ncu * per_cu) return false;
+  }
+  Scratch xch(ctx, sizeof(double) * 4 * size_t(n));
+  Scratch flags(ctx, 256);
+  unsigned* bar = flags.as<unsigned>();
+  unsigned* failp = bar + 32;
+  NSMI_HIP(hipMemsetAsync(flags.p, 0, 256, ctx->stream));
+  int nn = n;
+  double* xp = xch.as<double>();
+  void* args[] = {&A, &nn, &d, &e, &tau, &xp, &bar, &failp};
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 4.0 / 3.0 * double(n) * n * n, 0.0);
+    if (G > kPlainLaunchMax) {
+      const hipError_t st =
+          hipLaunchCooperativeKernel(p.fn, dim3(unsigned(G)), dim3(kTriThreads), args, 0, ctx->stream);
+      if (st != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+      }
+    } else {
+      NSMI_HIP(hipLaunchKernel(p.fn, dim3(unsigned(G)), dim3(kTriThreads), args, 0, ctx->stream));
+    }
+  }
+  unsigned f = 0;
+  NSMI_HIP(hipMemcpyAsync(&f, failp, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  if (f) fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
+  return true;
+}
+
+}  // namespace nsmi

@@ -103,7 +103,7 @@ __global__ void k_gather_rows(const double* __restrict__ X, int64_t ld, const in
   const int j = int(t / len);
   const int64_t i = t % len;
   const int src = idx ? idx[j] : j;
-  double f = sgn[j];
+  double f = sgn ? sgn[j] : 1.0;
   if (scale) f *= scale[j];
   out[t] = f * X[int64_t(src) * ld + i];
 }
@@ -225,9 +225,10 @@ struct FactCore {
   int route;
   int64_t iso = 0, oth = 0, full = 0;
   bool gram = false;
+  bool tri = false;  // rho holds sytrd reflectors, z the tridiagonal eigenvectors
   std::vector<double> P;
   std::vector<int> order;
-  std::unique_ptr<Scratch> rho, sv, ub, vb;
+  std::unique_ptr<Scratch> rho, sv, ub, vb, z, tau;
   FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
                                                                          route(r) {
     iso = rt ? nc : m;
@@ -253,14 +254,31 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
       else
         gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
     }
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
-      check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
-                                       f.rho->as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(),
-                                       ctx->d_info),
-                      "rocsolver_dsyevd");
+    // Tridiagonal route: the cooperative reduction, then rocSOLVER's divide and
+    // conquer on T; the back-transformation is deferred to fact_emit, where only
+    // the kept vectors are transformed.  Otherwise the library's dsyevd.
+    f.tau.reset(new Scratch(ctx, sizeof(double) * size_t(std::max<int64_t>(iso, 2))));
+    f.tri = sytrd_coop(ctx, f.rho->as<double>(), int(iso), w.as<double>(), e.as<double>(), f.tau->as<double>());
+    if (f.tri) {
+      f.z.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
+      {
+        ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 4.0 * double(iso) * iso * iso, 0.0);
+        check_rocsolver(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, rocblas_int(iso), w.as<double>(),
+                                         e.as<double>(), f.z->as<double>(), rocblas_int(iso), ctx->d_info),
+                        "rocsolver_dstedc");
+      }
+      check_info(ctx, "rocsolver_dstedc");
+    } else {
+      f.tau.reset();
+      {
+        ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
+        check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
+                                         f.rho->as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(),
+                                         ctx->d_info),
+                        "rocsolver_dsyevd");
+      }
+      check_info(ctx, "rocsolver_dsyevd");
     }
-    check_info(ctx, "rocsolver_dsyevd");
     std::vector<double> wh(iso);
     NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
     NSMI_HIP(hipStreamSynchronize(ctx->stream));
@@ -317,17 +335,37 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
   if (k == 0) return;
   const int64_t m = f.m, nc = f.nc, iso = f.iso, full = f.full;
   Scratch d_sgn(ctx, sizeof(double) * size_t(std::max<int64_t>(k, 1)));
+  Scratch d_idx(ctx, sizeof(int) * size_t(f.gram ? k : 1));
+  // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
+  const double* vecs = f.gram ? f.rho->as<double>() : nullptr;
+  const int* vidx = d_idx.as<int>();
+  std::unique_ptr<Scratch> zk;
+  if (f.gram) {
+    NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    if (f.tri) {
+      // kept eigenvectors of T (columns order[0..k) of z), then Q . Zk with the reflectors
+      ProfScope ps2(ctx, NSMI_K_FACT_SOLVE, 4.0 * double(iso) * iso * k, 0.0);
+      zk.reset(new Scratch(ctx, sizeof(double) * size_t(k) * iso));
+      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, f.z->as<double>(), iso,
+                         vidx, int(k), iso, (const double*)nullptr, (const double*)nullptr, zk->as<double>());
+      NSMI_LAUNCH_CHECK();
+      if (iso > 1)
+        check_rocsolver(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                         rocblas_int(iso), rocblas_int(k), f.rho->as<double>(), rocblas_int(iso),
+                                         f.tau->as<double>(), zk->as<double>(), rocblas_int(iso)),
+                        "rocsolver_dormtr");
+      vecs = zk->as<double>();
+      vidx = nullptr;
+    }
+  }
   ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(k) * m * nc, 8.0 * (double(m) * nc + double(k) * (m + nc)));
   if (f.gram) {
-    Scratch d_idx(ctx, sizeof(int) * size_t(k));
-    NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
-    // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
-    hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.rho->as<double>(), iso, iso, 1,
-                       d_idx.as<int>(), d_sgn.as<double>());
+    hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, vecs, iso, iso, 1, vidx,
+                       d_sgn.as<double>());
     NSMI_LAUNCH_CHECK();
     Scratch Ut(ctx, sizeof(double) * size_t(k) * iso);
-    hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, f.rho->as<double>(), iso,
-                       d_idx.as<int>(), int(k), iso, d_sgn.as<double>(), (const double*)nullptr, Ut.as<double>());
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, vecs, iso, vidx, int(k), iso,
+                       d_sgn.as<double>(), (const double*)nullptr, Ut.as<double>());
     NSMI_LAUNCH_CHECK();
     if (!f.rows_trailing) {
       // U (m x k) = Ut^T ; C (k x nc) = Ut . M

@@ -204,6 +204,14 @@ void mix_f32(nsmi_ctx* ctx, const float* X, float* Y, const double* Mx, int64_t 
 
 void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jacobi, ascending
 
+// ---- tridiagonal reduction (eig.hip) ----------------------------------------
+// Householder reduction of a symmetric n x n device matrix to tridiagonal form in
+// LAPACK's dsytrd "lower" layout (one cooperative launch); false when n is outside
+// [2, kSytrdCoopMax] or the device cannot co-schedule the grid.
+constexpr int kSytrdCoopMax = 1024;
+int sytrd_coop_groups(int n);
+bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau);
+
 // complex factorize / QR (zfactorize.hip); the truncation rule is passed in
 void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing, const nsmi_trunc* trunc,
                 int route, int64_t maxdim, int64_t mindim, nsmi_tensor** Uout, nsmi_tensor** Cout, int64_t* kept,
