@@ -30,15 +30,20 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=3, help="timed sweeps at chi = maxdim")
-    p.add_argument("--warmup", type=int, default=8, help="untimed ramp sweeps (maxdim 64, 256, 1024, ...)")
+    p.add_argument("--workload", default="chain", choices=["chain", "qn", "tree-tdvp"],
+                   help="chain: dense XXZ N=100 DMRG (the headline); qn: U(1) spin-1 N=64 block-sparse DMRG; "
+                        "tree-tdvp: 2-site TDVP quench of the TFIM on a 63-site binary tree")
+    p.add_argument("--steps", type=int, default=None,
+                   help="timed sweeps (chain, qn: default 3) or time steps (tree-tdvp: default 20)")
+    p.add_argument("--warmup", type=int, default=None,
+                   help="untimed ramp sweeps (chain, qn: default 8; tree-tdvp: 0)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one 256 MiB device-to-device tensor copy first (counter calibration under rocprofv3)")
     p.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3", "bf16"])
     p.add_argument("--nsites", type=int, default=100)
-    p.add_argument("--maxdim", type=int, default=1024)
+    p.add_argument("--maxdim", type=int, default=None, help="chain, qn: 1024; tree-tdvp: 256")
     p.add_argument("--delta", type=float, default=1.0)
-    p.add_argument("--cutoff", type=float, default=1e-12)
+    p.add_argument("--cutoff", type=float, default=None, help="chain: 1e-12; qn, tree-tdvp: 1e-10")
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-fp64-plain", action="store_true", help="skip the FP64-plain comparison sweep")
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -46,7 +51,17 @@ def parse():
     p.add_argument("--serving-requests", type=int, default=40,
                    help="Poisson-stream requests of the serving leg (0 skips it)")
     p.add_argument("--serving-slots", type=int, default=8)
-    return p.parse_args()
+    a = p.parse_args()
+    tree = a.workload == "tree-tdvp"
+    if a.steps is None:
+        a.steps = 20 if tree else 3
+    if a.warmup is None:
+        a.warmup = 0 if tree else 8
+    if a.maxdim is None:
+        a.maxdim = 256 if tree else 1024
+    if a.cutoff is None:
+        a.cutoff = 1e-12 if a.workload == "chain" else 1e-10
+    return a
 
 
 def ramp(maxdim, warmup):
@@ -98,7 +113,9 @@ def gpu_sweeps(args):
     print(f"bench: warm-up {len(sched)} sweeps {sched} in {t_warm:.1f} s, maxlinkdim {prob.state.maxlinkdim()}",
           file=sys.stderr, flush=True)
     linkdims_start = prob.state.linkdims()
-    # timed sweeps (HIP events around every profiled kernel, on the context stream)
+    # timed sweeps: HIP events (on the context stream) around the H_eff GEMM
+    # launches only, for the roofline of the dominant kernel
+    ctx.profile_classes(["heff_gemm1", "heff_gemm2"])
     ctx.profile(True)
     ctx.profile_reset()
     ctx.sync()
@@ -110,6 +127,7 @@ def gpu_sweeps(args):
     print(f"bench: timed window monotonic_ns {w0} {time.monotonic_ns()}", file=sys.stderr, flush=True)
     stats = ctx.profile_stats()
     ctx.profile(False)
+    ctx.profile_classes(None)
     E_mixed = ns.expect_mpo(psi, H)
     print(f"bench: {args.steps} timed sweeps ({args.precision}) in {t_timed:.2f} s, E={E_mixed:.14f}",
           file=sys.stderr, flush=True)
@@ -117,16 +135,23 @@ def gpu_sweeps(args):
            "linkdims_start": linkdims_start, "linkdims_end": psi.linkdims(), "regions": len(log),
            "mps_host": psi.to_arrays() if not args.no_cpu_baseline else None,
            "kept_mean": sum(r[2] for r in log) / max(len(log), 1)}
+    # one more sweep of the same form with events around every profiled launch:
+    # the time shares come from it (events on every launch slow the host loop)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    run_sweeps(prob, 1, args.maxdim, args.cutoff, args.precision)
+    ctx.sync()
+    out["t_profiled"] = time.perf_counter() - t0
+    out["stats_all"] = ctx.profile_stats()
+    ctx.profile(False)
     if not args.no_fp64_plain and args.precision != "fp64":
-        ctx.profile(True)
-        ctx.profile_reset()
         ctx.sync()
         t0 = time.perf_counter()
         _, psi64, _ = run_sweeps(prob, 1, args.maxdim, args.cutoff, "fp64")
         ctx.sync()
         out["t_fp64_sweep"] = time.perf_counter() - t0
-        out["stats_fp64"] = ctx.profile_stats()
-        ctx.profile(False)
         out["E_fp64"] = ns.expect_mpo(psi64, H)
         print(f"bench: 1 FP64-plain sweep in {out['t_fp64_sweep']:.2f} s, E={out['E_fp64']:.14f}",
               file=sys.stderr, flush=True)
@@ -203,12 +228,145 @@ def serving_leg(args):
             "peak_concurrency": srv.peak_concurrency}
 
 
+def _common(args, metric, value, unit, total_ms, steps, dtype, config, roofline, share):
+    return {"metric": metric, "value": round(value, 6), "unit": unit, "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": round(total_ms / max(steps, 1), 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config,
+            "roofline": roofline, "time_share": share}
+
+
+def _roofline(stats, cls, total_ms, peak=PEAK_TFLOPS["fp64"]):
+    s = stats[cls]
+    achieved = s["flops"] / (s["total_ms"] * 1e-3) / 1e12 if s["total_ms"] > 0 else 0.0
+    share = {k: round(v["total_ms"] / total_ms, 4) for k, v in stats.items() if v["launches"]}
+    return ({"bound": "mfma", "kernel": cls, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+             "frac": round(achieved / peak, 4), "traffic": None, "launches": s["launches"],
+             "avg_ms": round(s["total_ms"] / max(s["launches"], 1), 4),
+             "flops_per_launch": s["flops"] / max(s["launches"], 1)}, share)
+
+
+def qn_workload(args):
+    """Block-sparse U(1) DMRG: spin-1 Heisenberg chain N=64, MPO D=5, total Sz=0
+    from a Neel product state; untimed 2-site sweeps on the maxdim ramp 32, 64,
+    128, 256, 512, then maxdim, then the timed sweeps at maxdim (FP64 block
+    programs, Lanczos K=3)."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import models, qn
+    from networksolvers_amd.sweep import eigsolve_problem
+    ctx = ns.Context(0)
+    N = 64
+    site = qn.spin_site_index(3)
+    H = qn.QNMPO(ctx, models.heisenberg_mpo(N, d=3), site)
+    states = [2 if i % 2 == 0 else 0 for i in range(N)]
+    prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site), operator=ns.sweep.projected_operator(H))
+    sched = ([m for m in (32, 64, 128, 256, 512) if m < args.maxdim] + [args.maxdim] * args.warmup)[:args.warmup]
+
+    def sweeps(n, maxdim):
+        t = [time.perf_counter()]
+
+        def progress(region_iter, *, sweep, nsweeps, **kw):
+            p = region_iter.problem
+            now = time.perf_counter()
+            print(f"bench:   qn sweep {sweep}/{nsweeps} {now - t[0]:.2f} s  E={p.eigenvalue:.12f}  "
+                  f"maxlinkdim={p.state.maxlinkdim()}", file=sys.stderr, flush=True)
+            t[0] = now
+        return eigsolve_problem(prob, nsweeps=n, nsites=2, sweep_printer=progress,
+                                inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=maxdim)))
+
+    if sched:
+        sweeps(len(sched), sched)
+    ctx.sync()
+    t0 = time.perf_counter()
+    E, psi = sweeps(args.steps, args.maxdim)
+    ctx.sync()
+    t_timed = time.perf_counter() - t0
+    nsec = max(len(psi.links[i]) for i in range(N + 1))
+    E_psi = ns.expect_mpo(psi, H)
+    # one more sweep with per-launch HIP events for the roofline and time shares
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    sweeps(1, args.maxdim)
+    ctx.sync()
+    t_prof = time.perf_counter() - t0
+    stats = ctx.profile_stats()
+    ctx.profile(False)
+    ctx.close()
+    total_ms = 1e3 * t_timed
+    roof, share = _roofline(stats, "block", 1e3 * t_prof)
+    config = {"workload": f"2-site U(1) block-sparse DMRG, spin-1 Heisenberg chain N={N}, d=3, MPO D=5, total Sz=0 "
+                          f"(Neel start), maxdim {args.maxdim}, cutoff {args.cutoff:g}, Lanczos K=3",
+              "precision": "fp64", "regions_per_sweep": 2 * (N - 1), "maxlinkdim_timed": psi.maxlinkdim(),
+              "max_sectors_per_link": nsec, "energy": E_psi, "parallelism": "single GPU"}
+    return _common(args, "QN DMRG sweeps/s (spin-1 N=64, Sz=0)", args.steps / t_timed, "sweeps/s", total_ms,
+                   args.steps, "f64", config, roof, share)
+
+
+def tree_tdvp_workload(args):
+    """2-site TDVP quench, order 4, exponentiate solver: H = -sum ZZ - h sum X
+    (h = 1) on the complete binary tree of 63 sites, all-up start, dt = 0.05;
+    the timed steps are the first ``steps`` time steps (default 20, t = 1)."""
+    import numpy as np
+    import networksolvers_amd as ns
+    ctx = ns.Context(0)
+    n = 63
+    V = list(range(n))
+    G = ns.Graph(V, [((i - 1) // 2, i) for i in range(1, n)])
+    H = ns.TreeTTNO(ctx, G, ns.tfim_tree_mpo(G, h=1.0))
+    psi0 = ns.product_ttn(ctx, G, {v: 0 for v in V}, 2)
+    dt = 0.05
+    times = [dt * k for k in range(args.warmup + args.steps + 1)]
+    marks = []
+
+    def progress(region_iter, *, sweep, nsweeps, **kw):
+        ctx.sync()
+        marks.append(time.perf_counter())
+        print(f"bench:   tdvp step {sweep}/{nsweeps} t={dt * sweep:.2f}  maxlinkdim={region_iter.problem.state.maxlinkdim()}",
+              file=sys.stderr, flush=True)
+
+    kws = dict(nsites=2, tdvp_order=4, updater_kwargs=dict(solver=ns.exponentiate_solver),
+               inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=args.maxdim)))
+    t0 = time.perf_counter()
+    psi = ns.tdvp(H, psi0, times, sweep_printer=progress, **kws)
+    ctx.sync()
+    t_end = time.perf_counter()
+    t_start = marks[args.warmup - 1] if args.warmup > 0 else t0
+    t_timed = t_end - t_start
+    z = ns.expect_local(psi, np.diag([1.0, -1.0]))
+    leaves = [v for v in V if 2 * v + 1 >= n]
+    # one more time step from the final state with per-launch HIP events (roofline, time shares)
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    ns.tdvp(H, psi, [0.0, dt], sweep_printer=lambda *a, **k: None, **kws)
+    ctx.sync()
+    t_prof = time.perf_counter() - t0
+    stats = ctx.profile_stats()
+    ctx.profile(False)
+    ctx.close()
+    total_ms = 1e3 * t_timed
+    roof, share = _roofline(stats, "net", 1e3 * t_prof)
+    config = {"workload": f"2-site TDVP (order 4, Krylov exponentiate) quench of H=-sum ZZ - 1.0 sum X on the "
+                          f"complete binary tree of {n} sites, all-up start, dt {dt}, maxdim {args.maxdim}, "
+                          f"cutoff {args.cutoff:g}",
+              "precision": "complex128", "regions_per_step": 6 * (2 * (n - 1) - 1), "time": dt * len(marks),
+              "maxlinkdim_end": psi.maxlinkdim(),
+              "Z_root": float(np.real(z[0])), "Z_leaves_mean": float(np.mean([np.real(z[v]) for v in leaves])),
+              "parallelism": "single GPU"}
+    return _common(args, "TDVP time steps/s (63-site binary-tree TFIM, 2-site, order 4)", args.steps / t_timed,
+                   "steps/s", total_ms, args.steps, "c128", config, roof, share)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not served (the sweep "
                                    "would shard by MPO channel or real space; see the design notes)"}))
         sys.exit(2)
+    if args.workload != "chain":
+        line = qn_workload(args) if args.workload == "qn" else tree_tdvp_workload(args)
+        print(json.dumps(line), flush=True)
+        return
     r = gpu_sweeps(args)
     value = args.steps / r["t_timed"]
     st = r["stats"]
@@ -218,7 +376,8 @@ def main():
     achieved = s["flops"] / (s["total_ms"] * 1e-3) / 1e12 if s["total_ms"] > 0 else 0.0
     peak = PEAK_TFLOPS[args.precision]
     total_ms = 1e3 * r["t_timed"]
-    share = {k: round(v["total_ms"] / total_ms, 4) for k, v in st.items() if v["launches"]}
+    prof_ms = 1e3 * r["t_profiled"]
+    share = {k: round(v["total_ms"] / prof_ms, 4) for k, v in r["stats_all"].items() if v["launches"]}
     line = {
         "metric": "DMRG sweeps/s at chi=1024 (MFMA vs FP64 plain, dE<=1e-9); p50/p99 solve latency",
         "value": round(value, 6),
@@ -242,6 +401,7 @@ def main():
                      "launches": s["launches"], "avg_ms": round(s["total_ms"] / max(s["launches"], 1), 4),
                      "flops_per_launch": s["flops"] / max(s["launches"], 1)},
         "time_share": share,
+        "time_share_sweep_s": round(r["t_profiled"], 3),
         "energy": {"E_form": r["E_mixed"], "E_ritz_last_region": r["E_ritz"]},
     }
     if "t_fp64_sweep" in r:

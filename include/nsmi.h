@@ -111,7 +111,9 @@ typedef struct {
 #define NSMI_K_KRYLOV 5       /* Krylov vector kernels                         */
 #define NSMI_K_FACT_GEMM 6    /* Gram / projection GEMMs of factorize          */
 #define NSMI_K_FACT_SOLVE 7   /* dense eigensolver / SVD                       */
-#define NSMI_K_COUNT 8
+#define NSMI_K_BLOCK 8        /* block programs of QN (block-sparse) tensors   */
+#define NSMI_K_NET 9          /* contraction programs of tree regions          */
+#define NSMI_K_COUNT 10
 
 /* ---------------------------------------------------------------- context */
 int nsmi_version(void);
@@ -258,6 +260,9 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
 
 /* ------------------------------------------------------------- profiling */
 int nsmi_profile_enable(nsmi_ctx* ctx, int on);
+/* kernel classes recorded while profiling is on: bit k selects NSMI_K_* class k
+ * (default: all), so a timed run can carry events on the classes it reports only */
+int nsmi_profile_classes(nsmi_ctx* ctx, unsigned mask);
 int nsmi_profile_reset(nsmi_ctx* ctx);
 int nsmi_profile_query(nsmi_ctx* ctx, int kernel_class, nsmi_profile_stats* out);
 

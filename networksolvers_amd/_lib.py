@@ -21,7 +21,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2, "bf16": 3}
 ROUTE = {"auto": 0, "svd": 1, "eigen": 2}
 ROUTE_NAME = {1: "svd", 2: "eigen"}
 K_CLASSES = {"heff_gemm1": 0, "heff_gemm2": 1, "heff_mix": 2, "env": 3, "theta": 4, "krylov": 5,
-             "fact_gemm": 6, "fact_solve": 7}
+             "fact_gemm": 6, "fact_solve": 7, "block": 8, "net": 9}
 
 
 class LanczosOpts(C.Structure):
@@ -107,6 +107,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int), _PP]),
     "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),
     "nsmi_profile_reset": (C.c_int, [_VP]),
+    "nsmi_profile_classes": (C.c_int, [_VP, C.c_uint]),
     "nsmi_profile_query": (C.c_int, [_VP, C.c_int, C.POINTER(ProfileStats)]),
 }
 

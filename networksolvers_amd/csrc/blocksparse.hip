@@ -241,7 +241,7 @@ void bs_run(nsmi_ctx* ctx, BsProgram* p, const double* x, double* y) {
   bt.p[1] = y;
   for (int i = 0; i < p->n_operands; ++i) bt.p[2 + i] = static_cast<double*>(p->operands[i]->data);
   for (size_t i = 0; i < p->scratch.size(); ++i) bt.p[2 + p->n_operands + i] = static_cast<double*>(p->scratch[i]);
-  ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, p->flops, 0.0);
+  ProfScope ps(ctx, NSMI_K_BLOCK, p->flops, 0.0);
   for (const BsStep& s : p->steps) {
     if (s.nsched == 0) continue;
     if (s.kind == 0) {

@@ -123,7 +123,7 @@ static hipEvent_t take_event(nsmi_ctx* ctx) {
 }
 
 ProfScope::ProfScope(nsmi_ctx* c, int k, double f, double by) : ctx(c), cls(k), flops(f), bytes(by) {
-  if (!ctx->prof_on || k < 0) return;
+  if (!ctx->prof_on || k < 0 || k >= 32 || !((ctx->prof_mask >> k) & 1u)) return;
   a = take_event(ctx);
   b = take_event(ctx);
   NSMI_HIP(hipEventRecord(a, ctx->stream));
@@ -379,6 +379,13 @@ int nsmi_profile_enable(nsmi_ctx* ctx, int on) {
   API_BEGIN
   if (!on) drain_profile(ctx);
   ctx->prof_on = on != 0;
+  API_END(ctx)
+}
+
+int nsmi_profile_classes(nsmi_ctx* ctx, unsigned mask) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  ctx->prof_mask = mask;
   API_END(ctx)
 }
 

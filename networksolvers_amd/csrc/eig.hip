@@ -277,6 +277,115 @@ TriPlan tri_plan(int n) {
   return {reinterpret_cast<const void*>(k_sytrd_coop<16, 4, false>), 16};
 }
 
+// ---- batched Jacobi eigensolver for small Gram matrices ---------------------
+//
+// One workgroup per symmetric matrix (n <= NP), matrix and eigenvector
+// accumulator resident in LDS.  Cyclic two-sided Jacobi in round-robin
+// ("chess tournament") order: the m = n + (n odd) indices form m/2 disjoint
+// pairs per round (a padded index is a zero row and column, so it never
+// rotates), m - 1 rounds cover every pair once per sweep.  The rotations of a
+// round commute, so they are applied together: columns of A and V, then rows of
+// A.  A pair rotates when |a_pq| > eps ||A||_F (the Frobenius norm is invariant),
+// with the classical angle (t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)),
+// theta = (a_qq - a_pp) / (2 a_pq)), and its a_pq is then set to zero.  A sweep
+// without a rotation ends the solve.  Output in LAPACK dsyevd's layout: the
+// eigenvectors overwrite A column-major (column j = vector of eigenvalue w[j]),
+// eigenvalues unsorted.
+struct JacobiItem {
+  double* a;  // n x n, symmetric; overwritten by the eigenvectors
+  double* w;  // n eigenvalues
+  int n, pad;
+};
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_jacobi_eig(const JacobiItem* __restrict__ items, int max_sweeps,
+                                                    unsigned* __restrict__ unconverged) {
+  __shared__ double A[NP][NP + 1];
+  __shared__ double V[NP][NP + 1];
+  __shared__ double cs[NP / 2], sn[NP / 2];
+  __shared__ int pp[NP / 2], qq[NP / 2];
+  __shared__ double red[4];
+  __shared__ int s_rot;
+  const JacobiItem it = items[blockIdx.x];
+  const int n = it.n, m = n + (n & 1), half = m / 2, tid = threadIdx.x;
+  double nrm = 0.0;
+  for (int e = tid; e < m * m; e += 256) {
+    const int i = e / m, j = e % m;
+    const double v = (i < n && j < n) ? it.a[size_t(i) * n + j] : 0.0;
+    A[i][j] = v;
+    V[i][j] = (i == j) ? 1.0 : 0.0;
+    nrm += v * v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) nrm += __shfl_down(nrm, o, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = nrm;
+  __syncthreads();
+  const double tol = 2.220446049250313e-16 * sqrt(red[0] + red[1] + red[2] + red[3]);
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    if (tid == 0) s_rot = 0;
+    __syncthreads();
+    for (int r = 0; r + 1 < m; ++r) {
+      if (tid < half) {
+        const int a = tid == 0 ? r : (r + tid) % (m - 1);
+        const int b = tid == 0 ? m - 1 : (r - tid + m - 1) % (m - 1);
+        const int p = a < b ? a : b, q = a < b ? b : a;
+        const double apq = A[p][q];
+        double c = 1.0, s = 0.0;
+        if (fabs(apq) > tol) {
+          const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+          double t = fabs(theta) > 1e150 ? 0.5 / fabs(theta) : 1.0 / (fabs(theta) + sqrt(theta * theta + 1.0));
+          if (theta < 0.0) t = -t;
+          c = 1.0 / sqrt(t * t + 1.0);
+          s = t * c;
+          s_rot = 1;
+        }
+        pp[tid] = p;
+        qq[tid] = q;
+        cs[tid] = c;
+        sn[tid] = s;
+      }
+      __syncthreads();
+      // columns p, q of A and V (items: pair k, row i; NP is a power of two)
+      for (int e = tid; e < half * NP; e += 256) {
+        const int k = e / NP, i = e % NP;
+        const double s = sn[k];
+        if (i >= m || s == 0.0) continue;
+        const double c = cs[k];
+        const int p = pp[k], q = qq[k];
+        const double aip = A[i][p], aiq = A[i][q];
+        A[i][p] = c * aip - s * aiq;
+        A[i][q] = s * aip + c * aiq;
+        const double vip = V[i][p], viq = V[i][q];
+        V[i][p] = c * vip - s * viq;
+        V[i][q] = s * vip + c * viq;
+      }
+      __syncthreads();
+      // rows p, q of A (items: pair k, column j); the rotated pair's a_pq, a_qp become 0
+      for (int e = tid; e < half * NP; e += 256) {
+        const int k = e / NP, j = e % NP;
+        const double s = sn[k];
+        if (j >= m || s == 0.0) continue;
+        const double c = cs[k];
+        const int p = pp[k], q = qq[k];
+        const double apj = A[p][j], aqj = A[q][j];
+        A[p][j] = j == q ? 0.0 : c * apj - s * aqj;
+        A[q][j] = j == p ? 0.0 : s * apj + c * aqj;
+      }
+      __syncthreads();
+    }
+    const int rot = s_rot;
+    __syncthreads();
+    if (!rot) break;
+  }
+  if (sweep == max_sweeps && tid == 0) atomicOr(unconverged, 1u);
+  for (int e = tid; e < n * n; e += 256) {
+    const int j = e / n, i = e % n;
+    it.a[e] = V[i][j];
+  }
+  for (int i = tid; i < n; i += 256) it.w[i] = A[i][i];
+}
+
 }  // namespace
 
 int sytrd_coop_groups(int n) {
@@ -336,6 +445,43 @@ bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* t
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
   if (f) fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
   return true;
+}
+
+// Eigen-decomposition of every listed matrix (n <= kJacobiMax) in one launch.
+void jacobi_eig_batched(nsmi_ctx* ctx, const std::vector<JacobiJob>& jobs) {
+  if (jobs.empty()) return;
+  int nmax = 0;
+  std::vector<JacobiItem> h(jobs.size());
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    require(jobs[i].n >= 1 && jobs[i].n <= kJacobiMax, "jacobi: matrix size outside the batched solver's range");
+    h[i] = JacobiItem{jobs[i].a, jobs[i].w, jobs[i].n, 0};
+    nmax = std::max(nmax, jobs[i].n);
+  }
+  Scratch d_items(ctx, sizeof(JacobiItem) * h.size());
+  Scratch flag(ctx, 64);
+  NSMI_HIP(hipMemcpyAsync(d_items.p, h.data(), sizeof(JacobiItem) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+  NSMI_HIP(hipMemsetAsync(flag.p, 0, 64, ctx->stream));
+  constexpr int kMaxSweeps = 60;
+  double fl = 0.0;
+  for (const auto& j : jobs) fl += 9.0 * double(j.n) * j.n * j.n;
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_SOLVE, fl, 0.0);
+    const int m = nmax + (nmax & 1);
+    const JacobiItem* it = d_items.as<JacobiItem>();
+    unsigned* f = flag.as<unsigned>();
+    const dim3 grid(unsigned(h.size())), block(256);
+    if (m <= 16)
+      hipLaunchKernelGGL(k_jacobi_eig<16>, grid, block, 0, ctx->stream, it, kMaxSweeps, f);
+    else if (m <= 32)
+      hipLaunchKernelGGL(k_jacobi_eig<32>, grid, block, 0, ctx->stream, it, kMaxSweeps, f);
+    else
+      hipLaunchKernelGGL(k_jacobi_eig<64>, grid, block, 0, ctx->stream, it, kMaxSweeps, f);
+    NSMI_LAUNCH_CHECK();
+  }
+  unsigned fh = 0;
+  NSMI_HIP(hipMemcpyAsync(&fh, flag.p, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  if (fh) fail(NSMI_ERR_SOLVER, "jacobi: a Gram matrix did not converge within the sweep limit");
 }
 
 }  // namespace nsmi

@@ -225,10 +225,11 @@ struct FactCore {
   int route;
   int64_t iso = 0, oth = 0, full = 0;
   bool gram = false;
-  bool tri = false;  // rho holds sytrd reflectors, z the tridiagonal eigenvectors
+  bool tri = false;     // rho holds sytrd reflectors, z the tridiagonal eigenvectors
+  bool jacobi = false;  // rho awaits the batched Jacobi solve (eigenvalues into wdev)
   std::vector<double> P;
   std::vector<int> order;
-  std::unique_ptr<Scratch> rho, sv, ub, vb, z, tau;
+  std::unique_ptr<Scratch> rho, sv, ub, vb, z, tau, wdev;
   FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
                                                                          route(r) {
     iso = rt ? nc : m;
@@ -237,7 +238,26 @@ struct FactCore {
   }
 };
 
-void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
+// Spectrum P in NDTensors' order from the host copy of the eigenvalues.
+void fact_spectrum(FactCore& f, std::vector<double>& wh) {
+  const int64_t iso = f.iso;
+  for (double v : wh)
+    if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
+  f.order.resize(iso);
+  std::iota(f.order.begin(), f.order.end(), 0);
+  if (f.route == NSMI_ROUTE_EIGEN) {
+    // NDTensors eigen: sort by decreasing |lambda|; truncate! zeroes a negative tail
+    std::stable_sort(f.order.begin(), f.order.end(), [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
+  } else {
+    // svd semantics: P = s^2 >= 0 in decreasing order
+    for (auto& v : wh) v = std::max(v, 0.0);
+    std::stable_sort(f.order.begin(), f.order.end(), [&](int a, int b) { return wh[a] > wh[b]; });
+  }
+  f.P.resize(iso);
+  for (int64_t i = 0; i < iso; ++i) f.P[i] = wh[f.order[i]];
+}
+
+void fact_gram(nsmi_ctx* ctx, FactCore& f) {
   // rocSOLVER's dgesvd is ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs
   // 47 ms), so above a small size the "svd" route also diagonalises the Gram
   // matrix; its spectrum P = s^2 is then resolved to eps * P_max absolutely.
@@ -245,8 +265,6 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
   const int64_t iso = f.iso, oth = f.oth, m = f.m, nc = f.nc;
   if (f.gram) {
     f.rho.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
-    Scratch w(ctx, sizeof(double) * size_t(iso));
-    Scratch e(ctx, sizeof(double) * size_t(iso));
     {
       ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
       if (!f.rows_trailing)
@@ -254,6 +272,14 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
       else
         gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
     }
+    if (iso <= kJacobiMax) {
+      // small Gram matrices go to the batched Jacobi solver (fact_jacobi)
+      f.jacobi = true;
+      f.wdev.reset(new Scratch(ctx, sizeof(double) * size_t(iso)));
+      return;
+    }
+    Scratch w(ctx, sizeof(double) * size_t(iso));
+    Scratch e(ctx, sizeof(double) * size_t(iso));
     // Tridiagonal route: the cooperative reduction, then rocSOLVER's divide and
     // conquer on T; the back-transformation is deferred to fact_emit, where only
     // the kept vectors are transformed.  Otherwise the library's dsyevd.
@@ -282,21 +308,7 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
     std::vector<double> wh(iso);
     NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
     NSMI_HIP(hipStreamSynchronize(ctx->stream));
-    for (double v : wh)
-      if (!std::isfinite(v)) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite eigenvalue");
-    f.order.resize(iso);
-    std::iota(f.order.begin(), f.order.end(), 0);
-    if (f.route == NSMI_ROUTE_EIGEN) {
-      // NDTensors eigen: sort by decreasing |lambda|; truncate! zeroes a negative tail
-      std::stable_sort(f.order.begin(), f.order.end(),
-                       [&](int a, int b) { return std::fabs(wh[a]) > std::fabs(wh[b]); });
-    } else {
-      // svd semantics: P = s^2 >= 0 in decreasing order
-      for (auto& v : wh) v = std::max(v, 0.0);
-      std::stable_sort(f.order.begin(), f.order.end(), [&](int a, int b) { return wh[a] > wh[b]; });
-    }
-    f.P.resize(iso);
-    for (int64_t i = 0; i < iso; ++i) f.P[i] = wh[f.order[i]];
+    fact_spectrum(f, wh);
   } else {
     // SVD of the column-major view M^T (nc x m): M^T = Ut S Vt  =>  M = Vt^T S Ut^T
     // row-major readings: buffer "vb" (full x m col-major) = left vectors of M as m x full rows;
@@ -326,6 +338,37 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
       f.P[i] = sh[i] * sh[i];
     }
   }
+}
+
+
+// The deferred small Gram matrices of several cores in one batched launch.
+void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
+  std::vector<JacobiJob> jobs;
+  for (FactCore* f : fs)
+    if (f->jacobi) jobs.push_back(JacobiJob{f->rho->as<double>(), f->wdev->as<double>(), int(f->iso)});
+  if (jobs.empty()) return;
+  jacobi_eig_batched(ctx, jobs);
+  std::vector<std::vector<double>> wh;
+  wh.reserve(jobs.size());
+  for (FactCore* f : fs)
+    if (f->jacobi) {
+      wh.emplace_back(size_t(f->iso));
+      NSMI_HIP(hipMemcpyAsync(wh.back().data(), f->wdev->p, sizeof(double) * f->iso, hipMemcpyDeviceToHost,
+                              ctx->stream));
+    }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  size_t j = 0;
+  for (FactCore* f : fs)
+    if (f->jacobi) {
+      fact_spectrum(*f, wh[j++]);
+      f->jacobi = false;
+      f->wdev.reset();
+    }
+}
+
+void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
+  fact_gram(ctx, f);
+  if (f.jacobi) fact_jacobi(ctx, {&f});
 }
 
 // U / C for the first k entries of the sorted spectrum (U m x k and C k x nc, or
@@ -625,12 +668,15 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
   require(route == NSMI_ROUTE_SVD || route == NSMI_ROUTE_EIGEN, "factorize_sectors: unknown route");
   std::vector<std::unique_ptr<FactCore>> fc;
   int64_t full = 0;
+  std::vector<FactCore*> fcp;
   for (int q = 0; q < n; ++q) {
     fc.emplace_back(new FactCore(static_cast<const double*>(Mall->data) + offsets[q], rows[q], cols[q],
                                  rows_trailing != 0, route));
-    fact_prepare(ctx, *fc.back());
+    fact_gram(ctx, *fc.back());
+    fcp.push_back(fc.back().get());
     full += fc.back()->full;
   }
+  fact_jacobi(ctx, fcp);  // every small sector's eigensolve in one launch
   auto key = [&](double v) { return route == NSMI_ROUTE_EIGEN ? std::fabs(v) : v; };
   struct Entry {
     double key, value;

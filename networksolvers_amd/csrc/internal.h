@@ -87,6 +87,7 @@ struct nsmi_ctx {
   int* d_info = nullptr;
   // profiling
   bool prof_on = false;
+  unsigned prof_mask = ~0u;  // classes recorded while prof_on
   std::vector<nsmi::ProfRec> recs;
   std::vector<hipEvent_t> spare_events;
   nsmi_profile_stats stats[NSMI_K_COUNT] = {};
@@ -211,6 +212,16 @@ void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jaco
 constexpr int kSytrdCoopMax = 1024;
 int sytrd_coop_groups(int n);
 bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau);
+// Batched Jacobi eigensolver for small symmetric matrices (one workgroup each, LDS
+// resident): eigenvectors overwrite a (column-major, dsyevd layout), eigenvalues
+// (unsorted) go to w.  Synchronises.
+constexpr int kJacobiMax = 40;
+struct JacobiJob {
+  double* a;
+  double* w;
+  int n;
+};
+void jacobi_eig_batched(nsmi_ctx* ctx, const std::vector<JacobiJob>& jobs);
 
 // complex factorize / QR (zfactorize.hip); the truncation rule is passed in
 void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing, const nsmi_trunc* trunc,

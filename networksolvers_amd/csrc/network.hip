@@ -212,7 +212,7 @@ void net_release(nsmi_ctx* ctx, NetProgram* p) {
 }
 
 void net_apply(nsmi_ctx* ctx, NetProgram* p, const double* x, double* y) {
-  ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, p->flops, 0.0);
+  ProfScope ps(ctx, NSMI_K_NET, p->flops, 0.0);
   const void* cur = x;
   const int ns = int(p->steps.size());
   for (int i = 0; i < ns; ++i) {

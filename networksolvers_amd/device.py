@@ -84,6 +84,11 @@ class Context:
     def profile_reset(self):
         self.check(self.lib.nsmi_profile_reset(self.h))
 
+    def profile_classes(self, names=None):
+        """Record only these kernel classes while profiling is on (None: all)."""
+        mask = 0xFFFFFFFF if names is None else sum(1 << _lib.K_CLASSES[n] for n in names)
+        self.check(self.lib.nsmi_profile_classes(self.h, mask))
+
     def profile_stats(self):
         out = {}
         for name, k in _lib.K_CLASSES.items():
