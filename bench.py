@@ -357,7 +357,71 @@ def tree_tdvp_workload(args):
                    "steps/s", total_ms, args.steps, "c128", config, roof, share)
 
 
+def summarize_rocprof(root, window_log=None, top=25):
+    """Text summary of a rocprofv3 output folder (-f csv): the --stats table, the
+    dispatches inside the bench's timed window (bounds printed to stderr as
+    "timed window monotonic_ns A B"; rocprofv3 timestamps share that clock), and
+    per-kernel means of PMC counters.  Prints the summary."""
+    import collections
+    import csv
+    import glob
+    window = None
+    if window_log and os.path.exists(window_log):
+        for line in open(window_log):
+            if "timed window monotonic_ns" in line:
+                a, b = line.split()[-2:]
+                window = (int(a), int(b))
+    out = []
+    for f in glob.glob(os.path.join(root, "**", "*kernel_stats.csv"), recursive=True):
+        rows = sorted(csv.DictReader(open(f)), key=lambda r: -float(r.get("TotalDurationNs", 0) or 0))
+        out.append(f"== {os.path.basename(f)} (all dispatches of the command, top {top} by total time)")
+        if rows:
+            out.append(",".join(rows[0].keys()))
+        out += [",".join(str(v) for v in r.values()) for r in rows[:top]]
+    for f in glob.glob(os.path.join(root, "**", "*kernel_trace.csv"), recursive=True):
+        agg = collections.defaultdict(lambda: [0, 0])
+        n = 0
+        for r in csv.DictReader(open(f)):
+            n += 1
+            st, en = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+            if window and window[0] <= st <= window[1]:
+                a = agg[r.get("Kernel_Name", "?")]
+                a[0] += 1
+                a[1] += en - st
+        out.append(f"== {os.path.basename(f)}: {n} dispatches; timed window {window}")
+        if agg:
+            out.append("-- dispatches inside the timed window (name, calls, total_ms, avg_us)")
+            for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+                out.append(f"{name[:150]}, {c}, {t / 1e6:.3f}, {t / c / 1e3:.3f}")
+    for f in glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True):
+        agg = collections.defaultdict(lambda: [0, 0.0, 0.0])
+        aggw = collections.defaultdict(lambda: [0, 0.0])
+        for r in csv.DictReader(open(f)):
+            key = (r.get("Kernel_Name", "?"), r.get("Counter_Name", "?"))
+            a = agg[key]
+            v = float(r.get("Counter_Value", 0) or 0)
+            a[0] += 1
+            a[1] += v
+            a[2] = max(a[2], v)
+            st = int(r.get("Start_Timestamp", 0) or 0)
+            if window and window[0] <= st <= window[1]:
+                aggw[key][0] += 1
+                aggw[key][1] += v
+        out.append(f"== {os.path.basename(f)}")
+        out.append("-- kernel, counter, dispatches, mean value per dispatch, largest value of one dispatch")
+        for (k, c), (nn, sm, mx) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+            out.append(f"{k[:150]}, {c}, {nn}, {sm / nn:.1f}, {mx:.1f}")
+        if aggw:
+            out.append(f"-- dispatches inside the timed window {window} (kernel, counter, dispatches, mean value)")
+            for (k, c), (nn, sm) in sorted(aggw.items(), key=lambda kv: -kv[1][1]):
+                out.append(f"{k[:150]}, {c}, {nn}, {sm / nn:.1f}")
+    print("\n".join(out), flush=True)
+
+
 def main():
+    if len(sys.argv) > 2 and sys.argv[1] == "--summarize-rocprof":
+        summarize_rocprof(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
+        return
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not served (the sweep "
@@ -399,7 +463,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
                      "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                      "launches": s["launches"], "avg_ms": round(s["total_ms"] / max(s["launches"], 1), 4),
-                     "flops_per_launch": s["flops"] / max(s["launches"], 1)},
+                     "flops_per_launch": s["flops"] / max(s["launches"], 1),
+                     "algorithmic_bytes_per_launch": s["bytes"] / max(s["launches"], 1)},
         "time_share": share,
         "time_share_sweep_s": round(r["t_profiled"], 3),
         "energy": {"E_form": r["E_mixed"], "E_ritz_last_region": r["E_ritz"]},

@@ -122,16 +122,34 @@ static hipEvent_t take_event(nsmi_ctx* ctx) {
   return e;
 }
 
-ProfScope::ProfScope(nsmi_ctx* c, int k, double f, double by) : ctx(c), cls(k), flops(f), bytes(by) {
+ProfScope::ProfScope(nsmi_ctx* c, int k, double f, double by, bool kt)
+    : ctx(c), cls(k), flops(f), bytes(by), kernel_timed(kt) {
   if (!ctx->prof_on || k < 0 || k >= 32 || !((ctx->prof_mask >> k) & 1u)) return;
   a = take_event(ctx);
   b = take_event(ctx);
-  NSMI_HIP(hipEventRecord(a, ctx->stream));
+  if (kernel_timed) {
+    ctx->kev_start = a;
+    ctx->kev_stop = b;
+    ctx->kev_used = false;
+  } else {
+    NSMI_HIP(hipEventRecord(a, ctx->stream));
+  }
 }
 
 ProfScope::~ProfScope() {
   if (!a) return;
-  if (hipEventRecord(b, ctx->stream) != hipSuccess) return;
+  if (kernel_timed) {
+    const bool used = ctx->kev_used;
+    ctx->kev_start = ctx->kev_stop = nullptr;
+    ctx->kev_used = false;
+    if (!used) {  // nothing was launched inside the scope
+      ctx->spare_events.push_back(a);
+      ctx->spare_events.push_back(b);
+      return;
+    }
+  } else if (hipEventRecord(b, ctx->stream) != hipSuccess) {
+    return;
+  }
   ctx->recs.push_back(ProfRec{a, b, cls, flops, bytes});
 }
 

@@ -172,13 +172,13 @@ void launch_tile(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T>& a) {
   dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n), 1, unsigned(d.batch));
   dim3 block(256);
   if (!d.transA && !d.transB)
-    hipLaunchKernelGGL((k_gemm_mfma<T, BM, BN, false, false, ROLE>), grid, block, 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, false, false, ROLE>), grid, block, a);
   else if (!d.transA && d.transB)
-    hipLaunchKernelGGL((k_gemm_mfma<T, BM, BN, false, true, ROLE>), grid, block, 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, false, true, ROLE>), grid, block, a);
   else if (d.transA && !d.transB)
-    hipLaunchKernelGGL((k_gemm_mfma<T, BM, BN, true, false, ROLE>), grid, block, 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, true, false, ROLE>), grid, block, a);
   else
-    hipLaunchKernelGGL((k_gemm_mfma<T, BM, BN, true, true, ROLE>), grid, block, 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, true, true, ROLE>), grid, block, a);
   NSMI_LAUNCH_CHECK();
 }
 
@@ -343,9 +343,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_nt(BfArgs g) {
 template <int TERMS, typename TC>
 void bf16_role(nsmi_ctx* ctx, int role, dim3 grid, const BfArgs& a) {
   switch (role) {
-    case kRoleHeff1: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleHeff1>), grid, dim3(256), 0, ctx->stream, a); break;
-    case kRoleHeff2: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleHeff2>), grid, dim3(256), 0, ctx->stream, a); break;
-    default: hipLaunchKernelGGL((k_gemm_bf16_nt<TERMS, TC, kRoleEnv>), grid, dim3(256), 0, ctx->stream, a); break;
+    case kRoleHeff1: NSMI_LAUNCH_TIMED(ctx, (k_gemm_bf16_nt<TERMS, TC, kRoleHeff1>), grid, dim3(256), a); break;
+    case kRoleHeff2: NSMI_LAUNCH_TIMED(ctx, (k_gemm_bf16_nt<TERMS, TC, kRoleHeff2>), grid, dim3(256), a); break;
+    default: NSMI_LAUNCH_TIMED(ctx, (k_gemm_bf16_nt<TERMS, TC, kRoleEnv>), grid, dim3(256), a); break;
   }
 }
 

@@ -1,6 +1,7 @@
 // Internal declarations of libnsmi.so (not part of the C ABI).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -88,6 +89,9 @@ struct nsmi_ctx {
   // profiling
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // classes recorded while prof_on
+  // events a kernel-timed ProfScope hands to the next NSMI_LAUNCH_TIMED launch
+  hipEvent_t kev_start = nullptr, kev_stop = nullptr;
+  bool kev_used = false;
   std::vector<nsmi::ProfRec> recs;
   std::vector<hipEvent_t> spare_events;
   nsmi_profile_stats stats[NSMI_K_COUNT] = {};
@@ -122,14 +126,31 @@ struct Scratch {
 };
 
 // profiling brackets (no-ops when profiling is off)
+// With kernel_timed, the scope's events are not stream markers around the scope:
+// the first NSMI_LAUNCH_TIMED kernel inside it carries them (hipExtLaunchKernelGGL
+// records them at that kernel's own start and end), so the duration excludes the
+// dispatch gaps that markers around a short kernel would add.
 struct ProfScope {
   nsmi_ctx* ctx;
   int cls;
   double flops, bytes;
+  bool kernel_timed = false;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(nsmi_ctx* c, int k, double f, double by);
+  ProfScope(nsmi_ctx* c, int k, double f, double by, bool kernel_timed = false);
   ~ProfScope();
 };
+
+#define NSMI_LAUNCH_TIMED(ctx, KERNEL, grid, block, ...)                                                  \
+  do {                                                                                                    \
+    if ((ctx)->kev_start) {                                                                               \
+      hipExtLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, (ctx)->kev_start, (ctx)->kev_stop, 0, \
+                            __VA_ARGS__);                                                                 \
+      (ctx)->kev_start = (ctx)->kev_stop = nullptr;                                                       \
+      (ctx)->kev_used = true;                                                                             \
+    } else {                                                                                              \
+      hipLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, __VA_ARGS__);                             \
+    }                                                                                                     \
+  } while (0)
 
 // ---- dense kernels (gemm.hip) -------------------------------------------
 // Row-major C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]; op(A) is M x K

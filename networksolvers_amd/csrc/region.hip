@@ -338,12 +338,12 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     double* T1 = static_cast<double*>(op->T1);
     double* T3 = static_cast<double*>(op->T3);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, 4.0 * op->flops_gemm1, 16.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, 4.0 * op->flops_gemm1, 16.0 * (M1 * K1 + K1 * N1 + M1 * N1), true);
       gemm_c128(ctx, g1, dptr(op->L), false, x, false, T1, kRoleHeff1);
     }
     mix_f64(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, 2 * op->cr, NSMI_K_HEFF_MIX);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, 4.0 * op->flops_gemm2, 16.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, 4.0 * op->flops_gemm2, 16.0 * (M2 * K2 + K2 * N2 + M2 * N2), true);
       gemm_c128(ctx, g2, T3, false, dptr(op->R), false, y, kRoleHeff2);
     }
     return;
@@ -352,12 +352,12 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     double* T1 = static_cast<double*>(op->T1);
     double* T3 = static_cast<double*>(op->T3);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 8.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 8.0 * (M1 * K1 + K1 * N1 + M1 * N1), true);
       gemm_f64(ctx, g1, 1.0, dptr(op->L), x, 0.0, T1, kRoleHeff1);
     }
     mix_f64(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, op->cr, NSMI_K_HEFF_MIX);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 8.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 8.0 * (M2 * K2 + K2 * N2 + M2 * N2), true);
       gemm_f64(ctx, g2, 1.0, T3, dptr(op->R), 0.0, y, kRoleHeff2);
     }
     return;
@@ -368,12 +368,12 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     Scratch xs(ctx, sizeof(float) * op->n), ys(ctx, sizeof(float) * op->n);
     vec_to_f32(ctx, xs.as<float>(), x, op->n);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 4.0 * (M1 * K1 + K1 * N1 + M1 * N1));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 4.0 * (M1 * K1 + K1 * N1 + M1 * N1), true);
       gemm_f32(ctx, g1, op->L32, xs.as<float>(), T1, kRoleHeff1);
     }
     mix_f32(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, op->cr, NSMI_K_HEFF_MIX);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 4.0 * (M2 * K2 + K2 * N2 + M2 * N2));
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 4.0 * (M2 * K2 + K2 * N2 + M2 * N2), true);
       gemm_f32(ctx, g2, T3, op->R32, ys.as<float>(), kRoleHeff2);
     }
     vec_f32_to_f64(ctx, y, ys.as<float>(), op->n);
@@ -384,14 +384,14 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     const int terms = prec == NSMI_PREC_BF16X3 ? 3 : 1;
     split_bf16_pad(ctx, x, N1, K1, op->Kp1, op->xThi, op->xTlo, 1);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 2.0 * terms * (M1 + N1) * op->Kp1 + 4.0 * M1 * N1);
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 2.0 * terms * (M1 + N1) * op->Kp1 + 4.0 * M1 * N1, true);
       gemm_bf16_nt(ctx, M1, N1, op->Kp1, op->Lhi, op->Llo, op->Kp1, op->xThi, op->xTlo, op->Kp1, op->T1f, N1,
                    false, terms, kRoleHeff1);
     }
     mix_f32_to_bf16_rows(ctx, op->T1f, op->T3hi, op->T3lo, op->Mx, op->cl, op->P, op->Q, op->cr, int(op->D2),
                          op->Kp2, NSMI_K_HEFF_MIX);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 2.0 * terms * (M2 + N2) * op->Kp2 + 8.0 * M2 * N2);
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 2.0 * terms * (M2 + N2) * op->Kp2 + 8.0 * M2 * N2, true);
       gemm_bf16_nt(ctx, M2, N2, op->Kp2, op->T3hi, op->T3lo, op->Kp2, op->Rhi, op->Rlo, op->Kp2, y, N2, true, terms,
                    kRoleHeff2);
     }

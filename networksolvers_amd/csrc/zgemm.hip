@@ -153,13 +153,13 @@ void zlaunch(nsmi_ctx* ctx, const GemmDesc& d, ZArgs& a) {
   a.tiles_n = int((d.N + BN - 1) / BN);
   dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n));
   if (!d.transA && !d.transB)
-    hipLaunchKernelGGL((k_zgemm<BM, BN, false, false, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, false, false, ROLE>), grid, dim3(256), a);
   else if (!d.transA && d.transB)
-    hipLaunchKernelGGL((k_zgemm<BM, BN, false, true, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, false, true, ROLE>), grid, dim3(256), a);
   else if (d.transA && !d.transB)
-    hipLaunchKernelGGL((k_zgemm<BM, BN, true, false, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, true, false, ROLE>), grid, dim3(256), a);
   else
-    hipLaunchKernelGGL((k_zgemm<BM, BN, true, true, ROLE>), grid, dim3(256), 0, ctx->stream, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, true, true, ROLE>), grid, dim3(256), a);
   NSMI_LAUNCH_CHECK();
 }
 
