@@ -305,9 +305,15 @@ def qn_workload(args):
 def tree_tdvp_workload(args):
     """2-site TDVP quench, order 4, exponentiate solver: H = -sum ZZ - h sum X
     (h = 1) on the complete binary tree of 63 sites, all-up start, dt = 0.05;
-    the timed steps are the first ``steps`` time steps (default 20, t = 1)."""
+    the timed steps are the first ``steps`` time steps (default 20, t = 1).  The
+    timed form is --precision (ComplexF32 MFMA contraction programs with FP64
+    Krylov exponentiate for fp32); the ComplexF64-plain form is timed the same
+    way, and the fidelity and <Z_v> profile of the form are reported against it."""
     import numpy as np
     import networksolvers_amd as ns
+    from networksolvers_amd import tree as ptree
+    if args.precision not in ("fp64", "fp32"):
+        raise SystemExit("tree-tdvp serves --precision fp64 or fp32")
     ctx = ns.Context(0)
     n = 63
     V = list(range(n))
@@ -316,24 +322,37 @@ def tree_tdvp_workload(args):
     psi0 = ns.product_ttn(ctx, G, {v: 0 for v in V}, 2)
     dt = 0.05
     times = [dt * k for k in range(args.warmup + args.steps + 1)]
-    marks = []
-
-    def progress(region_iter, *, sweep, nsweeps, **kw):
-        ctx.sync()
-        marks.append(time.perf_counter())
-        print(f"bench:   tdvp step {sweep}/{nsweeps} t={dt * sweep:.2f}  maxlinkdim={region_iter.problem.state.maxlinkdim()}",
-              file=sys.stderr, flush=True)
-
-    kws = dict(nsites=2, tdvp_order=4, updater_kwargs=dict(solver=ns.exponentiate_solver),
-               inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=args.maxdim)))
-    t0 = time.perf_counter()
-    psi = ns.tdvp(H, psi0, times, sweep_printer=progress, **kws)
-    ctx.sync()
-    t_end = time.perf_counter()
-    t_start = marks[args.warmup - 1] if args.warmup > 0 else t0
-    t_timed = t_end - t_start
-    z = ns.expect_local(psi, np.diag([1.0, -1.0]))
+    Z = np.diag([1.0, -1.0])
     leaves = [v for v in V if 2 * v + 1 >= n]
+
+    def run(precision):
+        marks = []
+
+        def progress(region_iter, *, sweep, nsweeps, **kw):
+            ctx.sync()
+            marks.append(time.perf_counter())
+            print(f"bench:   tdvp ({precision}) step {sweep}/{nsweeps} t={dt * sweep:.2f}  "
+                  f"maxlinkdim={region_iter.problem.state.maxlinkdim()}", file=sys.stderr, flush=True)
+        kws = dict(nsites=2, tdvp_order=4, updater_kwargs=dict(solver=ns.exponentiate_solver, precision=precision),
+                   inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=args.maxdim)))
+        t0 = time.perf_counter()
+        psi = ns.tdvp(H, psi0, times, sweep_printer=progress, **kws)
+        ctx.sync()
+        t_end = time.perf_counter()
+        t_start = marks[args.warmup - 1] if args.warmup > 0 else t0
+        return psi, t_end - t_start, kws
+
+    psi, t_timed, kws = run(args.precision)
+    z = ns.expect_local(psi, Z)
+    plain = None
+    if args.precision != "fp64":
+        psi64, t64, _ = run("fp64")
+        z64 = ns.expect_local(psi64, Z)
+        ov = ptree.inner(psi64, psi)
+        fid = abs(ov) ** 2 / abs(ptree.inner(psi64, psi64) * ptree.inner(psi, psi))
+        plain = {"steps_per_s": round(args.steps / t64, 6), "speedup_of_form": round(t64 / t_timed, 3),
+                 "fidelity_vs_fp64": fid, "max_abs_dZ_vs_fp64": max(abs(z[v] - z64[v]) for v in V),
+                 "maxlinkdim_end_fp64": psi64.maxlinkdim()}
     # one more time step from the final state with per-launch HIP events (roofline, time shares)
     ctx.profile(True)
     ctx.profile_reset()
@@ -345,16 +364,21 @@ def tree_tdvp_workload(args):
     ctx.profile(False)
     ctx.close()
     total_ms = 1e3 * t_timed
-    roof, share = _roofline(stats, "net", 1e3 * t_prof)
+    peak = PEAK_TFLOPS[args.precision]
+    roof, share = _roofline(stats, "net", 1e3 * t_prof, peak)
     config = {"workload": f"2-site TDVP (order 4, Krylov exponentiate) quench of H=-sum ZZ - 1.0 sum X on the "
                           f"complete binary tree of {n} sites, all-up start, dt {dt}, maxdim {args.maxdim}, "
                           f"cutoff {args.cutoff:g}",
-              "precision": "complex128", "regions_per_step": 6 * (2 * (n - 1) - 1), "time": dt * len(marks),
+              "precision": args.precision, "regions_per_step": 6 * (2 * (n - 1) - 1), "time": dt * args.steps,
               "maxlinkdim_end": psi.maxlinkdim(),
               "Z_root": float(np.real(z[0])), "Z_leaves_mean": float(np.mean([np.real(z[v]) for v in leaves])),
               "parallelism": "single GPU"}
-    return _common(args, "TDVP time steps/s (63-site binary-tree TFIM, 2-site, order 4)", args.steps / t_timed,
-                   "steps/s", total_ms, args.steps, "c128", config, roof, share)
+    line = _common(args, "TDVP time steps/s (63-site binary-tree TFIM, 2-site, order 4)", args.steps / t_timed,
+                   "steps/s", total_ms, args.steps, {"fp64": "c128", "fp32": "c64+c128"}[args.precision], config,
+                   roof, share)
+    if plain:
+        line["complex128_plain"] = plain
+    return line
 
 
 def summarize_rocprof(root, window_log=None, top=25):

@@ -217,12 +217,14 @@ typedef struct nsmi_net_step {
  * region, operator_map.jl:3-10, with up to four incident environments): applying
  * it runs the steps on x and permutes the result by out_perm, which must give x's
  * shape.  The returned handle is used like nsmi_heff_create's (apply, rayleigh,
- * Lanczos, Krylov exponential, destroy); FP64 / C128 only.  Operands must stay
- * alive until nsmi_heff_destroy; they are read in [contracted..., free...] order,
- * permuted once here when needed. */
+ * Lanczos, Krylov exponential, destroy).  Operands (F64 or C128) must stay alive
+ * until nsmi_heff_destroy; they are read in [contracted..., free...] order,
+ * permuted once here when needed.  precision NSMI_PREC_FP32 runs the program in
+ * FP32 / ComplexF32 on FP32 MFMA (operands converted once here, x and y converted
+ * per apply; the Krylov vectors stay F64 / C128); NSMI_PREC_FP64 runs it as given. */
 int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_steps,
                       const nsmi_net_step* steps, int x_rank, const int64_t* x_dims, const int* out_perm,
-                      nsmi_heff** out);
+                      int precision, nsmi_heff** out);
 
 /* ------------------------------------------ quantum-number (block-sparse) */
 /* A block program: the block operations of one contraction of U(1) block-sparse

@@ -104,7 +104,7 @@ SIGNATURES = {
                                          C.POINTER(C.c_int64), C.c_int, C.POINTER(Trunc), _PP, _PP,
                                          C.POINTER(C.c_int64), C.POINTER(TruncInfo)]),
     "nsmi_netop_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.c_int, C.POINTER(NetStep), C.c_int,
-                                    C.POINTER(C.c_int64), C.POINTER(C.c_int), _PP]),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int), C.c_int, _PP]),
     "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),
     "nsmi_profile_reset": (C.c_int, [_VP]),
     "nsmi_profile_classes": (C.c_int, [_VP, C.c_uint]),

@@ -149,7 +149,7 @@ def _(T: ApplyExpProblem, local_state, region_iter, *, nsites, time_step, solver
         if nxt is not None and list(nxt) != list(curr):
             (v,) = curr
             if isinstance(state(T), tree.TreeTTN):
-                local_state = tree.back_evolve_bond(T, v, list(nxt), local_state, time_step, solver, kws)
+                local_state = tree.back_evolve_bond(T, v, list(nxt), local_state, time_step, solver, kws, precision)
             else:
                 local_state = _back_evolve_bond(T, v, nxt[0] > v, local_state, time_step, solver, precision, kws)
     T.current_time = T.current_time + time_step

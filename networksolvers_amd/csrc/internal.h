@@ -216,6 +216,9 @@ void vec_zlincomb(nsmi_ctx* ctx, double* out, const double* const* V, const doub
 // complex GEMM (zgemm.hip): C = op(A) op(B), interleaved complex, optional conjugation
 void gemm_c128(nsmi_ctx* ctx, const GemmDesc& d, const double* A, bool conjA, const double* B, bool conjB,
                double* C, int role);
+// complex FP32 (interleaved float pairs) on FP32 MFMA, FP32 accumulation
+void gemm_c64(nsmi_ctx* ctx, const GemmDesc& d, const float* A, bool conjA, const float* B, bool conjB, float* C,
+              int role);
 
 // ---- MPO mixing (region.hip) --------------------------------------------
 // Y[o][p][i] = sum_q Mx[p][q] X[o][q][i]

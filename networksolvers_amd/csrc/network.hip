@@ -4,7 +4,7 @@
 // to four environments, the region's MPO tensors and the state (operator_map.jl:3-10
 // with ITensorNetworks' ProjTTN on a tree).  They are expressed with two device
 // primitives:
-//   permute    out[i_perm...] = in[i...]   (rank <= 8, F64 or interleaved C128)
+//   permute    out[i_perm...] = in[i...]   (rank <= 8; 4-, 8- and 16-byte elements)
 //   tensordot  C[free(A)..., free(B)...] = sum op(A) op(B) over paired axes, run as
 //              one MFMA GEMM; an operand is permuted into matrix form first unless
 //              its axes already read as a row-major (or transposed) matrix.
@@ -91,6 +91,9 @@ void permute_dev(nsmi_ctx* ctx, size_t elt, const void* in, int rank, const int6
   if (elt == 16)
     hipLaunchKernelGGL(k_permute<c16>, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<const c16*>(in),
                        static_cast<c16*>(out), a);
+  else if (elt == 4)
+    hipLaunchKernelGGL(k_permute<float>, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<const float*>(in),
+                       static_cast<float*>(out), a);
   else
     hipLaunchKernelGGL(k_permute<double>, dim3(blocks), dim3(256), 0, ctx->stream, static_cast<const double*>(in),
                        static_cast<double*>(out), a);
@@ -158,10 +161,15 @@ MatView plan_view(int rank, const int64_t* dims, int n, const int* axes, bool fr
   return v;
 }
 
-void gemm_any(nsmi_ctx* ctx, bool cplx, const GemmDesc& g, const void* A, bool conjA, const void* B, bool conjB,
-              void* C, int role) {
+void gemm_any(nsmi_ctx* ctx, bool cplx, bool lo, const GemmDesc& g, const void* A, bool conjA, const void* B,
+              bool conjB, void* C, int role) {
   if (g.M == 0 || g.N == 0) return;
-  if (cplx)
+  if (lo && cplx)
+    gemm_c64(ctx, g, static_cast<const float*>(A), conjA, static_cast<const float*>(B), conjB, static_cast<float*>(C),
+             role);
+  else if (lo)
+    gemm_f32(ctx, g, static_cast<const float*>(A), static_cast<const float*>(B), static_cast<float*>(C), role);
+  else if (cplx)
     gemm_c128(ctx, g, static_cast<const double*>(A), conjA, static_cast<const double*>(B), conjB,
               static_cast<double*>(C), role);
   else
@@ -190,7 +198,11 @@ struct NetStepPlan {
 
 struct NetProgram {
   bool cplx = false;
-  size_t elt = 8;
+  bool lo = false;     // FP32 form: operands, intermediates and the GEMMs in FP32 / ComplexF32
+  size_t elt = 8;      // element size of x, y and the given operands
+  size_t celt = 8;     // element size the program computes in
+  void* xlo = nullptr; // x and y in the FP32 form
+  void* ylo = nullptr;
   std::vector<NetStepPlan> steps;
   int out_perm[kMaxRank] = {};
   bool out_identity = true;
@@ -208,12 +220,20 @@ void net_release(nsmi_ctx* ctx, NetProgram* p) {
   scratch_release(ctx, p->buf[0]);
   scratch_release(ctx, p->buf[1]);
   scratch_release(ctx, p->pbuf);
+  scratch_release(ctx, p->xlo);
+  scratch_release(ctx, p->ylo);
   delete p;
 }
 
 void net_apply(nsmi_ctx* ctx, NetProgram* p, const double* x, double* y) {
   ProfScope ps(ctx, NSMI_K_NET, p->flops, 0.0);
+  const int64_t scalars = p->n * (p->cplx ? 2 : 1);
+  void* yout = p->lo ? p->ylo : static_cast<void*>(y);
   const void* cur = x;
+  if (p->lo) {
+    vec_to_f32(ctx, static_cast<float*>(p->xlo), x, scalars);
+    cur = p->xlo;
+  }
   const int ns = int(p->steps.size());
   for (int i = 0; i < ns; ++i) {
     const NetStepPlan& s = p->steps[i];
@@ -223,18 +243,19 @@ void net_apply(nsmi_ctx* ctx, NetProgram* p, const double* x, double* y) {
     } else if (s.xv.trans) {
       ta = true;
     } else {
-      permute_dev(ctx, p->elt, cur, s.rank_in, s.dims_in, s.xv.perm, p->pbuf);
+      permute_dev(ctx, p->celt, cur, s.rank_in, s.dims_in, s.xv.perm, p->pbuf);
       a = p->pbuf;
     }
-    void* out = (i == ns - 1 && p->out_identity) ? static_cast<void*>(y) : p->buf[i & 1];
-    gemm_any(ctx, p->cplx, mdesc(s.M, s.N, s.K, ta, false), a, false, s.opd, s.conj, out, kRoleHeff2);
+    void* out = (i == ns - 1 && p->out_identity) ? yout : p->buf[i & 1];
+    gemm_any(ctx, p->cplx, p->lo, mdesc(s.M, s.N, s.K, ta, false), a, false, s.opd, s.conj, out, kRoleHeff2);
     cur = out;
   }
   if (ns == 0 || !p->out_identity) {
     const int rank = ns ? p->steps.back().rank_out : p->rank_x;
     const int64_t* dims = ns ? p->steps.back().dims_out : p->dims_x;
-    permute_dev(ctx, p->elt, cur, rank, dims, p->out_perm, y);
+    permute_dev(ctx, p->celt, cur, rank, dims, p->out_perm, yout);
   }
+  if (p->lo) vec_f32_to_f64(ctx, y, static_cast<const float*>(p->ylo), scalars);
 }
 
 int64_t net_size(const NetProgram* p) { return p->n; }
@@ -320,8 +341,8 @@ int nsmi_tensordot(nsmi_ctx* ctx, const nsmi_tensor* A, int conj_a, const nsmi_t
     const int64_t K = va.cols;
     const double f = (cplx ? 8.0 : 2.0) * double(va.rows) * double(vb.cols) * double(K);
     ProfScope ps(ctx, NSMI_K_ENV, f, double(elt) * double(A->numel + B->numel + t->numel));
-    gemm_any(ctx, cplx, mdesc(va.rows, vb.cols, K, va.trans, vb.trans), pa, conj_a != 0, pb, conj_b != 0, t->data,
-             kRoleEnv);
+    gemm_any(ctx, cplx, false, mdesc(va.rows, vb.cols, K, va.trans, vb.trans), pa, conj_a != 0, pb, conj_b != 0,
+             t->data, kRoleEnv);
   } catch (...) {
     tensor_delete(ctx, t);
     throw;
@@ -332,11 +353,12 @@ int nsmi_tensordot(nsmi_ctx* ctx, const nsmi_tensor* A, int conj_a, const nsmi_t
 
 int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_steps,
                       const nsmi_net_step* steps, int x_rank, const int64_t* x_dims, const int* out_perm,
-                      nsmi_heff** out) {
+                      int precision, nsmi_heff** out) {
   if (!ctx || !out) return NSMI_ERR_INVALID;
   NetProgram* p = nullptr;
   API_BEGIN
   *out = nullptr;
+  require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32, "netop: precision must be FP64 or FP32");
   require(n_operands >= 0 && n_steps >= 0 && (n_steps == 0 || (operands && steps)), "netop: bad operand/step list");
   require(x_rank >= 1 && x_rank <= kMaxRank && x_dims && out_perm, "netop: bad x description");
   p = new NetProgram();
@@ -352,6 +374,9 @@ int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* o
   if (dtype < 0) dtype = NSMI_F64;
   p->cplx = dtype == NSMI_C128;
   p->elt = dtype_size(dtype);
+  p->lo = precision == NSMI_PREC_FP32;
+  p->celt = p->lo ? p->elt / 2 : p->elt;
+  const int64_t comp = p->cplx ? 2 : 1;  // scalars per element
   int rank = x_rank;
   int64_t dims[kMaxRank];
   std::copy(x_dims, x_dims + x_rank, dims);
@@ -376,7 +401,18 @@ int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* o
     s.K = s.xv.cols;
     s.N = ov.cols;
     s.conj = st.conj != 0 && p->cplx;
-    if (ov.direct) {
+    if (p->lo) {
+      // FP32 copy of the operand in [contracted..., free...] order
+      Scratch tmp(ctx, ov.direct ? 0 : O->bytes);
+      const void* src = O->data;
+      if (!ov.direct) {
+        permute_dev(ctx, p->elt, O->data, O->rank, O->dims, ov.perm, tmp.p);
+        src = tmp.p;
+      }
+      s.owned = scratch(ctx, p->celt * size_t(O->numel));
+      vec_to_f32(ctx, static_cast<float*>(s.owned), static_cast<const double*>(src), O->numel * comp);
+      s.opd = s.owned;
+    } else if (ov.direct) {
       s.opd = O->data;
     } else {
       s.owned = scratch(ctx, O->bytes);
@@ -401,10 +437,14 @@ int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* o
     require(dims[out_perm[k]] == x_dims[k], "netop: the program does not map x's space to itself");
   p->out_identity = is_identity(rank, out_perm);
   if (max_buf) {
-    p->buf[0] = scratch(ctx, p->elt * max_buf);
-    p->buf[1] = scratch(ctx, p->elt * max_buf);
+    p->buf[0] = scratch(ctx, p->celt * max_buf);
+    p->buf[1] = scratch(ctx, p->celt * max_buf);
   }
-  if (max_perm) p->pbuf = scratch(ctx, p->elt * max_perm);
+  if (max_perm) p->pbuf = scratch(ctx, p->celt * max_perm);
+  if (p->lo) {
+    p->xlo = scratch(ctx, p->celt * size_t(p->n));
+    p->ylo = scratch(ctx, p->celt * size_t(p->n));
+  }
   *out = heff_wrap_net(p);
   p = nullptr;
   }

@@ -332,6 +332,24 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
   const int64_t M2 = op->cl * op->dphys, K2 = op->D2 * op->cr, N2 = op->cr;
   GemmDesc g1 = desc(M1, N1, K1, false, false);
   GemmDesc g2 = desc(M2, N2, K2, false, true);
+  if (op->cplx && prec == NSMI_PREC_FP32) {
+    // ComplexF32 form: x in, y out as ComplexF64; GEMMs on FP32 MFMA
+    float* T1 = static_cast<float*>(op->T1);
+    float* T3 = static_cast<float*>(op->T3);
+    Scratch xs(ctx, sizeof(float) * 2 * op->n), ys(ctx, sizeof(float) * 2 * op->n);
+    vec_to_f32(ctx, xs.as<float>(), x, 2 * op->n);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, 4.0 * op->flops_gemm1, 8.0 * (M1 * K1 + K1 * N1 + M1 * N1), true);
+      gemm_c64(ctx, g1, op->L32, false, xs.as<float>(), false, T1, kRoleHeff1);
+    }
+    mix_f32(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, 2 * op->cr, NSMI_K_HEFF_MIX);
+    {
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, 4.0 * op->flops_gemm2, 8.0 * (M2 * K2 + K2 * N2 + M2 * N2), true);
+      gemm_c64(ctx, g2, T3, false, op->R32, false, ys.as<float>(), kRoleHeff2);
+    }
+    vec_f32_to_f64(ctx, y, ys.as<float>(), 2 * op->n);
+    return;
+  }
   if (op->cplx) {
     // complex GEMMs; the real MPO mixing acts on interleaved (re, im) pairs as on
     // a real array with a doubled innermost extent
@@ -467,7 +485,8 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
     check_f64(R, "R");
   } else {
     require(R->dtype == NSMI_C128, "heff: L and R must have the same element type");
-    if (precision != NSMI_PREC_FP64) fail(NSMI_ERR_UNSUPPORTED, "heff: complex operators are served in FP64 only");
+    if (precision != NSMI_PREC_FP64 && precision != NSMI_PREC_FP32)
+      fail(NSMI_ERR_UNSUPPORTED, "heff: complex operators are served in FP64 and FP32");
   }
   check_f64(W1, "W1 (the MPO is real)");
   require(L->rank == 3 && R->rank == 3 && W1->rank == 4, "heff: L, R must be rank 3 and W rank 4");
@@ -516,7 +535,7 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   const bool bf = precision == NSMI_PREC_BF16X3 || precision == NSMI_PREC_BF16;
   const bool two = precision == NSMI_PREC_BF16X3;
   if (!bf) {
-    const size_t elt = cplx ? 16 : prec_elt(precision);
+    const size_t elt = (cplx ? 2 : 1) * prec_elt(precision);
     op->T1 = scratch(ctx, elt * size_t(op->cl * op->D0 * op->dphys * op->cr));
     op->T3 = scratch(ctx, elt * size_t(op->cl * op->D2 * op->dphys * op->cr));
   } else {
@@ -546,10 +565,11 @@ int nsmi_heff_create(nsmi_ctx* ctx, const nsmi_tensor* L, const nsmi_tensor* W1,
   op->flops_gemm2 = 2.0 * double(op->cl * op->dphys) * double(op->cr) * double(op->D2 * op->cr);
   op->flops_mix = 2.0 * double(op->cl) * double(op->cr) * op->P * op->Q;
   if (precision == NSMI_PREC_FP32) {
-    op->L32 = static_cast<float*>(scratch(ctx, sizeof(float) * L->numel));
-    op->R32 = static_cast<float*>(scratch(ctx, sizeof(float) * R->numel));
-    vec_to_f32(ctx, op->L32, dptr(L), L->numel);
-    vec_to_f32(ctx, op->R32, dptr(R), R->numel);
+    const int64_t c = cplx ? 2 : 1;  // scalars per element
+    op->L32 = static_cast<float*>(scratch(ctx, sizeof(float) * L->numel * c));
+    op->R32 = static_cast<float*>(scratch(ctx, sizeof(float) * R->numel * c));
+    vec_to_f32(ctx, op->L32, dptr(L), L->numel * c);
+    vec_to_f32(ctx, op->R32, dptr(R), R->numel * c);
   }
   *out = op;
   op = nullptr;
@@ -591,27 +611,18 @@ int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, doubl
   API_BEGIN
   require(x && x->dtype == (op->cplx ? NSMI_C128 : NSMI_F64), "rayleigh: element type does not match");
   require(x->numel == op->n, "rayleigh: size mismatch");
-  if (op->cplx) {
-    Scratch ys(ctx, 16 * size_t(op->n));
-    heff_apply_impl(ctx, op, static_cast<const double*>(x->data), ys.as<double>(), NSMI_PREC_FP64);
-    vec_zdot_dev(ctx, static_cast<const double*>(x->data), ys.as<double>(), op->n, 0);
-    vec_dot_dev(ctx, static_cast<const double*>(x->data), static_cast<const double*>(x->data), 2 * op->n, 2);
-    double r[3];
-    read_slots(ctx, 3, r);
-    require(r[2] > 0.0, "rayleigh: zero vector");
-    *e = r[0] / r[2];
-    return NSMI_OK;
-  }
-  // an FP64 apply needs FP64-sized workspace
+  // an FP64 apply of a reduced-precision operator needs FP64-sized workspace
+  const size_t e64 = op->cplx ? 16 : 8;
+  const bool swap = !op->net && !op->bs && op->precision != NSMI_PREC_FP64;
   void* T1 = op->T1;
   void* T3 = op->T3;
-  Scratch t1(ctx, op->precision == NSMI_PREC_FP64 ? 8 : 8 * size_t(op->cl * op->D0 * op->dphys * op->cr));
-  Scratch t3(ctx, op->precision == NSMI_PREC_FP64 ? 8 : 8 * size_t(op->cl * op->D2 * op->dphys * op->cr));
-  if (op->precision != NSMI_PREC_FP64) {
+  Scratch t1(ctx, swap ? e64 * size_t(op->cl * op->D0 * op->dphys * op->cr) : 8);
+  Scratch t3(ctx, swap ? e64 * size_t(op->cl * op->D2 * op->dphys * op->cr) : 8);
+  if (swap) {
     op->T1 = t1.p;
     op->T3 = t3.p;
   }
-  Scratch ys(ctx, 8 * size_t(op->n));
+  Scratch ys(ctx, e64 * size_t(op->n));
   try {
     heff_apply_impl(ctx, op, dptr(x), ys.as<double>(), NSMI_PREC_FP64);
   } catch (...) {
@@ -621,6 +632,15 @@ int nsmi_heff_rayleigh(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, doubl
   }
   op->T1 = T1;
   op->T3 = T3;
+  if (op->cplx) {
+    vec_zdot_dev(ctx, dptr(x), ys.as<double>(), op->n, 0);
+    vec_dot_dev(ctx, dptr(x), dptr(x), 2 * op->n, 2);
+    double r[3];
+    read_slots(ctx, 3, r);
+    require(r[2] > 0.0, "rayleigh: zero vector");
+    *e = r[0] / r[2];
+    return NSMI_OK;
+  }
   vec_dot_dev(ctx, dptr(x), ys.as<double>(), op->n, 0);
   vec_dot_dev(ctx, dptr(x), dptr(x), op->n, 1);
   double r[2];

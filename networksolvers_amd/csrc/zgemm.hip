@@ -1,12 +1,14 @@
-// Complex FP64 GEMM on v_mfma_f64_16x16x4 for the TDVP path (ComplexF64 states).
+// Complex GEMM for the TDVP path: ComplexF64 on v_mfma_f64_16x16x4 and ComplexF32
+// on v_mfma_f32_16x16x4 (the FP32 form of tree H_eff contraction programs).
 //
-// Complex tensors are interleaved (re, im) doubles.  Tiles are staged into LDS as
+// Complex tensors are interleaved (re, im) pairs.  Tiles are staged into LDS as
 // separate real and imaginary planes (conjugation applied at staging), and each
 // 16x16 output block takes four real MFMAs:
 //   re += ar.br - ai.bi      im += ar.bi + ai.br
 // Row-major, op(A) M x K (A stored K x M when transA), op(B) K x N (stored N x K
 // when transB); C = op(A) op(B) (no scaling).  Output rows of the f64 MFMA are
-// interleaved across lane groups: item r of lane l is row l/16 + 4r.
+// interleaved across lane groups (item r of lane l is row l/16 + 4r); the f32
+// MFMA writes row 4(l/16) + r.
 #include "internal.h"
 
 namespace nsmi {
@@ -14,11 +16,28 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
 
+template <typename T> struct ZT;
+template <> struct ZT<double> {
+  typedef d4 acc;
+  typedef d2 pair;
+  __device__ static d4 mfma(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kl, int r) { return kl + 4 * r; }
+};
+template <> struct ZT<float> {
+  typedef f4 acc;
+  typedef f2 pair;
+  __device__ static f4 mfma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kl, int r) { return 4 * kl + r; }
+};
+
+template <typename T>
 struct ZArgs {
-  const double* A;
-  const double* B;
-  double* C;
+  const T* A;
+  const T* B;
+  T* C;
   int64_t M, N, K, lda, ldb, ldc;  // leading dimensions in complex elements
   int conjA, conjB;
   int tiles_m, tiles_n;
@@ -30,23 +49,25 @@ __device__ inline int zxcd_remap(int wg, int nwg) {
   return (wg % kXcd) * (nwg / kXcd) + wg / kXcd;
 }
 
-template <int BM, int BN, bool TA, bool TB, int ROLE>
-__global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
+template <typename T, int BM, int BN, bool TA, bool TB, int ROLE>
+__global__ __launch_bounds__(256) void k_zgemm(ZArgs<T> g) {
+  typedef typename ZT<T>::acc Acc;
+  typedef typename ZT<T>::pair V2;
   constexpr int BK = 8, PAD = 4;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
-  __shared__ double sAr[2][BK][BM + PAD], sAi[2][BK][BM + PAD];
-  __shared__ double sBr[2][BK][BN + PAD], sBi[2][BK][BN + PAD];
+  __shared__ T sAr[2][BK][BM + PAD], sAi[2][BK][BM + PAD];
+  __shared__ T sBr[2][BK][BN + PAD], sBi[2][BK][BN + PAD];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int tile = zxcd_remap(blockIdx.x, g.tiles_m * g.tiles_n);
   const int tm = tile / g.tiles_n, tn = tile % g.tiles_n;
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
-  const double sa = g.conjA ? -1.0 : 1.0, sb = g.conjB ? -1.0 : 1.0;
+  const T sa = g.conjA ? T(-1) : T(1), sb = g.conjB ? T(-1) : T(1);
 
-  d2 ra[EA > 0 ? EA : 1], rb[EB > 0 ? EB : 1];
+  V2 ra[EA > 0 ? EA : 1], rb[EB > 0 ? EB : 1];
   auto load = [&](int64_t k0) {
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
@@ -54,9 +75,9 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
       const int mm = TA ? (e % BM) : (e / BK);
       const int kk = TA ? (e / BM) : (e % BK);
       const int64_t gm = m0 + mm, gk = k0 + kk;
-      d2 v = {0.0, 0.0};
+      V2 v = {T(0), T(0)};
       if (gm < g.M && gk < g.K)
-        v = *reinterpret_cast<const d2*>(g.A + 2 * (TA ? gk * g.lda + gm : gm * g.lda + gk));
+        v = *reinterpret_cast<const V2*>(g.A + 2 * (TA ? gk * g.lda + gm : gm * g.lda + gk));
       ra[j] = v;
     }
 #pragma unroll
@@ -65,9 +86,9 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
       const int nn = TB ? (e / BK) : (e % BN);
       const int kk = TB ? (e % BK) : (e / BN);
       const int64_t gn = n0 + nn, gk = k0 + kk;
-      d2 v = {0.0, 0.0};
+      V2 v = {T(0), T(0)};
       if (gn < g.N && gk < g.K)
-        v = *reinterpret_cast<const d2*>(g.B + 2 * (TB ? gn * g.ldb + gk : gk * g.ldb + gn));
+        v = *reinterpret_cast<const V2*>(g.B + 2 * (TB ? gn * g.ldb + gk : gk * g.ldb + gn));
       rb[j] = v;
     }
   };
@@ -90,13 +111,13 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
     }
   };
 
-  d4 acr[FM][FN], aci[FM][FN];
+  Acc acr[FM][FN], aci[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      acr[i][j] = d4{0, 0, 0, 0};
-      aci[i][j] = d4{0, 0, 0, 0};
+      acr[i][j] = Acc{0, 0, 0, 0};
+      aci[i][j] = Acc{0, 0, 0, 0};
     }
   const int64_t nk = (g.K + BK - 1) / BK;
   load(0);
@@ -108,7 +129,7 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
     if (kt + 1 < nk) load((kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
-      double ar[FM], ai[FM], br[FN], bi[FN];
+      T ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
         ar[i] = sAr[buf][ks * 4 + kl][wm * WTM + i * 16 + ml];
@@ -123,10 +144,10 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          acr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], br[j], acr[i][j], 0, 0, 0);
-          acr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[i], bi[j], acr[i][j], 0, 0, 0);
-          aci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[i], bi[j], aci[i][j], 0, 0, 0);
-          aci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[i], br[j], aci[i][j], 0, 0, 0);
+          acr[i][j] = ZT<T>::mfma(ar[i], br[j], acr[i][j]);
+          acr[i][j] = ZT<T>::mfma(-ai[i], bi[j], acr[i][j]);
+          aci[i][j] = ZT<T>::mfma(ar[i], bi[j], aci[i][j]);
+          aci[i][j] = ZT<T>::mfma(ai[i], br[j], aci[i][j]);
         }
     }
     if (kt + 1 < nk) store(buf ^ 1);
@@ -138,54 +159,64 @@ __global__ __launch_bounds__(256) void k_zgemm(ZArgs g) {
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm * WTM + i * 16 + kl + 4 * r;
+        const int64_t gm = m0 + wm * WTM + i * 16 + ZT<T>::row(kl, r);
         const int64_t gn = n0 + wn * WTN + j * 16 + ml;
         if (gm < g.M && gn < g.N) {
-          d2 v = {acr[i][j][r], aci[i][j][r]};
-          *reinterpret_cast<d2*>(g.C + 2 * (gm * g.ldc + gn)) = v;
+          V2 v = {acr[i][j][r], aci[i][j][r]};
+          *reinterpret_cast<V2*>(g.C + 2 * (gm * g.ldc + gn)) = v;
         }
       }
 }
 
-template <int BM, int BN, int ROLE>
-void zlaunch(nsmi_ctx* ctx, const GemmDesc& d, ZArgs& a) {
+template <typename T, int BM, int BN, int ROLE>
+void zlaunch(nsmi_ctx* ctx, const GemmDesc& d, ZArgs<T>& a) {
   a.tiles_m = int((d.M + BM - 1) / BM);
   a.tiles_n = int((d.N + BN - 1) / BN);
   dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n));
   if (!d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, false, false, ROLE>), grid, dim3(256), a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<T, BM, BN, false, false, ROLE>), grid, dim3(256), a);
   else if (!d.transA && d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, false, true, ROLE>), grid, dim3(256), a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<T, BM, BN, false, true, ROLE>), grid, dim3(256), a);
   else if (d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, true, false, ROLE>), grid, dim3(256), a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<T, BM, BN, true, false, ROLE>), grid, dim3(256), a);
   else
-    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<BM, BN, true, true, ROLE>), grid, dim3(256), a);
+    NSMI_LAUNCH_TIMED(ctx, (k_zgemm<T, BM, BN, true, true, ROLE>), grid, dim3(256), a);
   NSMI_LAUNCH_CHECK();
 }
 
-template <int ROLE>
-void zrole(nsmi_ctx* ctx, const GemmDesc& d, ZArgs& a) {
+template <typename T, int ROLE>
+void zrole(nsmi_ctx* ctx, const GemmDesc& d, ZArgs<T>& a) {
   const int64_t mid = ((d.M + 63) / 64) * ((d.N + 63) / 64);
   if (mid >= 256)
-    zlaunch<64, 64, ROLE>(ctx, d, a);
+    zlaunch<T, 64, 64, ROLE>(ctx, d, a);
   else
-    zlaunch<32, 32, ROLE>(ctx, d, a);
+    zlaunch<T, 32, 32, ROLE>(ctx, d, a);
+}
+
+template <typename T>
+void gemm_cplx(nsmi_ctx* ctx, const GemmDesc& d, const T* A, bool conjA, const T* B, bool conjB, T* C, int role) {
+  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch == 1, "zgemm: bad sizes (no batching)");
+  if (d.M == 0 || d.N == 0) return;
+  ZArgs<T> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, conjA ? 1 : 0, conjB ? 1 : 0, 0, 0};
+  switch (role) {
+    case kRoleHeff1: zrole<T, kRoleHeff1>(ctx, d, a); break;
+    case kRoleHeff2: zrole<T, kRoleHeff2>(ctx, d, a); break;
+    case kRoleEnv: zrole<T, kRoleEnv>(ctx, d, a); break;
+    case kRoleTheta: zrole<T, kRoleTheta>(ctx, d, a); break;
+    default: zrole<T, kRoleFact>(ctx, d, a); break;
+  }
 }
 
 }  // namespace
 
 void gemm_c128(nsmi_ctx* ctx, const GemmDesc& d, const double* A, bool conjA, const double* B, bool conjB,
                double* C, int role) {
-  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch == 1, "zgemm: bad sizes (no batching)");
-  if (d.M == 0 || d.N == 0) return;
-  ZArgs a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, conjA ? 1 : 0, conjB ? 1 : 0, 0, 0};
-  switch (role) {
-    case kRoleHeff1: zrole<kRoleHeff1>(ctx, d, a); break;
-    case kRoleHeff2: zrole<kRoleHeff2>(ctx, d, a); break;
-    case kRoleEnv: zrole<kRoleEnv>(ctx, d, a); break;
-    case kRoleTheta: zrole<kRoleTheta>(ctx, d, a); break;
-    default: zrole<kRoleFact>(ctx, d, a); break;
-  }
+  gemm_cplx<double>(ctx, d, A, conjA, B, conjB, C, role);
+}
+
+void gemm_c64(nsmi_ctx* ctx, const GemmDesc& d, const float* A, bool conjA, const float* B, bool conjB, float* C,
+              int role) {
+  gemm_cplx<float>(ctx, d, A, conjA, B, conjB, C, role);
 }
 
 }  // namespace nsmi

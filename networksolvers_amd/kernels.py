@@ -109,14 +109,18 @@ class NetMap(_OperatorBase):
 
     ``steps`` are (operand index, axes of the running tensor, axes of the operand,
     conj) tuples; after them the running tensor is permuted by ``out_perm`` back
-    to the layout of x (shape ``x_shape``).  Keeps its operands alive."""
+    to the layout of x (shape ``x_shape``).  ``precision`` "fp32" runs the program
+    in FP32 / ComplexF32 on FP32 MFMA with x and y converted per apply (the Krylov
+    vectors stay FP64).  Keeps its operands alive."""
 
-    def __init__(self, operands, steps, x_shape, out_perm):
+    def __init__(self, operands, steps, x_shape, out_perm, precision="fp64"):
         self.ctx = operands[0].ctx if operands else None
         if self.ctx is None:
             raise ValueError("NetMap needs at least one operand")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"tree contraction programs run in fp64 or fp32, not {precision!r}")
         self._keep = tuple(operands)
-        self.precision = "fp64"
+        self.precision = precision
         ops = (C.c_void_p * len(operands))(*[o.h.value for o in operands])
         st = (_lib.NetStep * max(len(steps), 1))()
         for i, (j, ax, ao, conj) in enumerate(steps):
@@ -132,7 +136,7 @@ class NetMap(_OperatorBase):
         perm = (C.c_int * len(out_perm))(*out_perm)
         h = C.c_void_p()
         self.ctx.check(self.ctx.lib.nsmi_netop_create(self.ctx.h, len(operands), ops, len(steps), st, len(x_shape),
-                                                      dims, perm, C.byref(h)))
+                                                      dims, perm, _prec(precision), C.byref(h)))
         self.h = h
 
 

@@ -234,10 +234,8 @@ def optimal_map(P: ProjTTN, psi: TTN, precision="fp64") -> kernels.HeffMap:
             raise ValueError("QN regions are served in FP64 only")
         return qn.heff_map(P, psi)
     if isinstance(P, tree.TreeProjTTN):
-        if precision != "fp64":
-            raise ValueError("tree regions are served in FP64 (ComplexF64) only")
         region = P.sites()
-        return P.netmap(psi, tree.region_labels(psi, region), tree.region_shape(psi, region))
+        return P.netmap(psi, tree.region_labels(psi, region), tree.region_shape(psi, region), precision)
     L, R = P.left_right(psi)
     sites = P.sites()
     H = P.operator()

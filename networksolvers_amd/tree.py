@@ -37,7 +37,7 @@ def _tdot(A, la, B, lb, conj_a=False, conj_b=False):
     return out, [x for x in la if x not in common] + [x for x in lb if x not in common]
 
 
-def _program(x_labels, x_shape, operands):
+def _program(x_labels, x_shape, operands, precision="fp64"):
     """A device contraction program from labels: ``operands`` is a list of
     (tensor, labels) contracted in order with the running tensor (starting as x)
     over their common labels; the result must carry the bra labels of x and is
@@ -50,7 +50,7 @@ def _program(x_labels, x_shape, operands):
     target = [_bra(x) for x in x_labels]
     if sorted(map(repr, cur)) != sorted(map(repr, target)):
         raise RuntimeError(f"contraction program does not close: {cur} vs {target}")
-    return kernels.NetMap([t for t, _ in operands], steps, list(x_shape), [cur.index(t) for t in target])
+    return kernels.NetMap([t for t, _ in operands], steps, list(x_shape), [cur.index(t) for t in target], precision)
 
 
 def _permute_to(A, la, target):
@@ -473,7 +473,7 @@ class TreeProjTTN:
             self.environment(psi, e)
         return self
 
-    def netmap(self, psi: TreeTTN, x_labels, x_shape) -> kernels.NetMap:
+    def netmap(self, psi: TreeTTN, x_labels, x_shape, precision="fp64") -> kernels.NetMap:
         """The region's effective Hamiltonian as a device contraction program:
         per region vertex its first incident environment, its MPO tensor, then the
         rest, so the running tensor carries at most two open MPO links."""
@@ -488,7 +488,7 @@ class TreeProjTTN:
                 e = tg.key(n, r)
                 envs.append((self.environment(psi, (n, r)), [("b", e), ("w", e), ("k", e)]))
             operands += envs[:1] + [(self.H.tensor(r, psi.dtype), tg.mpo_labels(r))] + envs[1:]
-        return _program(x_labels, x_shape, operands)
+        return _program(x_labels, x_shape, operands, precision)
 
 
 # ----------------------------------------------------------------- region step
@@ -545,7 +545,7 @@ def insert(psi: TreeTTN, region, local, trunc, which_decomp=None):
     return info
 
 
-def back_evolve_bond(T, v, nxt_region, x, time_step, solver, kws):
+def back_evolve_bond(T, v, nxt_region, x, time_step, solver, kws, precision="fp64"):
     """applyexp.jl:30-42 on a tree: the first edge (v, w) on the path to the next
     region; x = Q R over it, R evolved by -time_step with the bond H_eff (the
     environment of v's side rebuilt with Q, w's side as cached); returns Q R(t)."""
@@ -562,7 +562,7 @@ def back_evolve_bond(T, v, nxt_region, x, time_step, solver, kws):
     Ew = P.environment(psi, (w, v))                            # w's side: [b, w, k] over the old link
     q = ("q", e)
     op = _program([("k", q), ("k", e)], R.shape,
-                  [(Eq, [("b", q), ("w", e), ("k", q)]), (Ew, [("b", e), ("w", e), ("k", e)])])
+                  [(Eq, [("b", q), ("w", e), ("k", q)]), (Ew, [("b", e), ("w", e), ("k", e)])], precision)
     try:
         Rt, _info = solver(op, -time_step, R, **kws)
     finally:
@@ -587,6 +587,41 @@ def expect_mpo(psi: TreeTTN, H: TreeTTNO) -> float:
         op.close()
 
 
+def inner(a: TreeTTN, b: TreeTTN):
+    """<a|b> by a host contraction from the leaves toward the last vertex (each
+    edge carries a (bra link, ket link) matrix); for reports on small bond
+    dimensions, e.g. the fidelity between two forms of the same evolution."""
+    tg = a.tg
+    A, B = a.to_arrays(), b.to_arrays()
+    root = tg.vertices[-1]
+    parent, order, stack = {root: None}, [], [root]
+    while stack:                                   # DFS: parents before children
+        v = stack.pop()
+        order.append(v)
+        for n in tg.adj[v]:
+            if n not in parent:
+                parent[n] = v
+                stack.append(n)
+    msg = {}
+    letters = "abcdefghijklmnopqrstuvwxyz"
+    for v in reversed(order):                      # children before parents
+        k = len(tg.adj[v])
+        bra = letters[:k] + "Y"
+        ket = letters[k:2 * k] + "Y"
+        terms, ops = [bra, ket], [np.conj(A[v]), B[v]]
+        out = ""
+        for i, n in enumerate(tg.adj[v]):
+            if n == parent[v]:
+                out = letters[i] + letters[k + i]
+            else:
+                terms.append(letters[i] + letters[k + i])
+                ops.append(msg[(n, v)])
+        r = np.einsum(",".join(terms) + "->" + out, *ops)
+        if parent[v] is None:
+            return complex(r) if np.iscomplexobj(r) else float(r)
+        msg[(v, parent[v])] = r
+
+
 def expect_local(psi: TreeTTN, op) -> dict:
     """<psi|op_v|psi> / <psi|psi> for every vertex v (ITensorNetworks ``expect``
     with a one-site operator, as in the reference's quench example): the
@@ -606,4 +641,4 @@ def expect_local(psi: TreeTTN, op) -> dict:
 
 __all__ = ["TreeGraph", "TreeTTN", "TreeTTNO", "TreeProjTTN", "tree_mpo", "tfim_tree_mpo", "heisenberg_tree_mpo",
            "product_ttn", "random_ttn", "random_ttn_arrays", "orthogonalize", "extract", "insert",
-           "back_evolve_bond", "expect_mpo", "expect_local", "region_labels", "region_shape"]
+           "back_evolve_bond", "expect_mpo", "expect_local", "inner", "region_labels", "region_shape"]

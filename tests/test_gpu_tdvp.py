@@ -91,12 +91,31 @@ def test_heff1_c128(ctx, cl, cr, d, D):
     assert rel(H(ctx.upload(x)).numpy(), chain.heff1_apply(L, W, R, x)) < 1e-12
 
 
-def test_complex_precision_forms_rejected(ctx):
+@pytest.mark.parametrize("cl,cr,d,D", [(1, 1, 2, 1), (20, 33, 2, 5), (64, 64, 2, 5), (129, 130, 2, 3)])
+def test_heff2_c128_fp32_form(ctx, cl, cr, d, D):
+    """ComplexF32 form (operands and GEMMs in complex FP32 on FP32 MFMA, x and y
+    ComplexF64): within FP32 rounding of the oracle (2e-5 relative); its Rayleigh
+    quotient is evaluated in FP64 (1e-12)."""
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(17)
+    L, R = crand((cl, D, cl), 18), crand((cr, D, cr), 19)
+    W1, W2 = rng.standard_normal((D, D, d, d)), rng.standard_normal((D, D, d, d))
+    th = crand((cl, d, d, cr), 20)
+    H = ns.HeffMap(ctx.upload(L), ctx.upload(W1), ctx.upload(W2), ctx.upload(R), "fp32")
+    want = chain.heff2_apply(L, W1, W2, R, th)
+    x = ctx.upload(th)
+    for _ in range(2):   # the operator's workspace is reused across applies
+        assert rel(H(x).numpy(), want) < 2e-5
+    e = H.rayleigh(x)
+    assert abs(e - np.vdot(th, want).real / np.vdot(th, th).real) < 1e-12 * max(1.0, abs(e))
+
+
+def test_complex_bf16_forms_rejected(ctx):
     import networksolvers_amd as ns
     L, R = crand((4, 5, 4), 1), crand((4, 5, 4), 2)
     W = np.zeros((5, 5, 2, 2))
-    with pytest.raises(Exception, match="FP64"):
-        ns.HeffMap(ctx.upload(L), ctx.upload(W), ctx.upload(W), ctx.upload(R), "fp32")
+    with pytest.raises(Exception, match="FP64 and FP32"):
+        ns.HeffMap(ctx.upload(L), ctx.upload(W), ctx.upload(W), ctx.upload(R), "bf16x3")
 
 
 def _hermitian_region(N, chi, seed, site):

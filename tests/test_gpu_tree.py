@@ -160,6 +160,67 @@ def test_tree_env_and_heff_vs_oracle(ctx, cplx):
         op.close()
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_tree_heff_fp32_form(ctx, cplx):
+    """The FP32 / ComplexF32 form of every 1- and 2-vertex contraction program on
+    the 7-vertex binary tree equals the FP64 program to FP32 rounding (2e-5
+    relative), over repeated applies."""
+    from networksolvers_amd import tree as ptree
+    V, E = binary_tree(2)
+    T = otree.Tree(V, E)
+    G, H, psi = _device_tree(ctx, V, E, ptree.tfim_tree_mpo(__import__("networksolvers_amd").Graph(V, E), h=0.8),
+                             full_rank_random(T, 9, cplx), cplx)
+    P = ptree.TreeProjTTN(H)
+    rng = np.random.default_rng(4)
+    for region in [[0], [3], [1, 0], [3, 1], [2, 6]]:
+        P.position(psi, region)
+        shape = ptree.region_shape(psi, region)
+        x = ctx.upload(rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0))
+        op64 = P.netmap(psi, ptree.region_labels(psi, region), shape)
+        op32 = P.netmap(psi, ptree.region_labels(psi, region), shape, "fp32")
+        want = op64(x).numpy()
+        for _ in range(2):
+            assert rel(op32(x).numpy(), want) < 2e-5, region
+        op64.close()
+        op32.close()
+
+
+def test_c4_quench_fp32_form_vs_oracle(ctx):
+    """The configured tree quench in its ComplexF32-MFMA form (contraction programs
+    in ComplexF32, Krylov exponentiate and factorisation in FP64) for 4 steps
+    against the FP64 oracle: same bond dimensions, 1 - fidelity < 1e-8, <Z_v>
+    within 1e-5."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import tree as ptree
+    V, E = binary_tree(5)
+    T = otree.Tree(V, E)
+    G = ns.Graph(V, E)
+    W = ns.tfim_tree_mpo(G, h=1.0)
+    up = {}
+    for v in V:
+        a = np.zeros([1] * len(T.adj[v]) + [2])
+        a[(0,) * len(T.adj[v]) + (0,)] = 1.0
+        up[v] = a
+    H = ns.TreeTTNO(ctx, G, W)
+    times = [0.05 * k for k in range(5)]
+    out = ns.tdvp(H, ns.TreeTTN.from_arrays(ctx, G, up), times, nsites=2, tdvp_order=4,
+                  updater_kwargs=dict(solver=ns.exponentiate_solver, precision="fp32"),
+                  inserter_kwargs=dict(trunc=dict(cutoff=1e-10, maxdim=256)))
+    ref = otree.tree_applyexp(V, E, W, up, [-1j * t for t in times], nsites=2, tdvp_order=4, maxdim=256,
+                              cutoff=1e-10)
+    got = out.to_arrays()
+    for (a, b) in E:
+        assert got[a].shape[T.adj[a].index(b)] == ref.psi[a].shape[T.adj[a].index(b)]
+    fid = abs(otree.inner(T, ref.psi, got)) ** 2 / abs(otree.inner(T, ref.psi, ref.psi) * otree.inner(T, got, got))
+    assert 1 - fid < 1e-8
+    zg = ns.expect_local(out, np.diag([1.0, -1.0]))
+    zr = otree.local_expectations(T, ref.psi, np.diag([1.0, -1.0]))
+    assert max(abs(zg[v] - zr[v]) for v in V) < 1e-5
+    # the product's own overlap (used by the bench's form comparison) agrees with the oracle's
+    ref_dev = ns.TreeTTN.from_arrays(ctx, G, ref.psi, dtype=np.complex128)
+    assert abs(ptree.inner(ref_dev, out) - otree.inner(T, ref.psi, got)) < 1e-12 * abs(otree.inner(T, ref.psi, got))
+
+
 def test_tree_dmrg_reference_case(ctx):
     """test_tree_dmrg.jl:39-76 on the device: |E - E_ED| < 1e-5 with cutoff 1e-5,
     maxdim 40, 5 sweeps; and every region equal to the oracle (eigenvalue 1e-10
