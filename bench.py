@@ -248,8 +248,10 @@ def _roofline(stats, cls, total_ms, peak=PEAK_TFLOPS["fp64"]):
 def qn_workload(args):
     """Block-sparse U(1) DMRG: spin-1 Heisenberg chain N=64, MPO D=5, total Sz=0
     from a Neel product state; untimed 2-site sweeps on the maxdim ramp 32, 64,
-    128, 256, 512, then maxdim, then the timed sweeps at maxdim (FP64 block
-    programs, Lanczos K=3)."""
+    128, 256, 512, then maxdim, then the timed sweeps at maxdim (H_eff block
+    programs in --precision, environments and factorisation FP64, Lanczos K=3);
+    one further FP64-plain sweep from the same state gives the FP64 throughput and
+    |dE/E| between the forms (FP64 <psi|H|psi>)."""
     import networksolvers_amd as ns
     from networksolvers_amd import models, qn
     from networksolvers_amd.sweep import eigsolve_problem
@@ -261,7 +263,10 @@ def qn_workload(args):
     prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site), operator=ns.sweep.projected_operator(H))
     sched = ([m for m in (32, 64, 128, 256, 512) if m < args.maxdim] + [args.maxdim] * args.warmup)[:args.warmup]
 
-    def sweeps(n, maxdim):
+    if args.precision not in ("fp64", "fp32"):
+        raise SystemExit("qn serves --precision fp64 or fp32")
+
+    def sweeps(n, maxdim, precision=args.precision):
         t = [time.perf_counter()]
 
         def progress(region_iter, *, sweep, nsweeps, **kw):
@@ -271,6 +276,7 @@ def qn_workload(args):
                   f"maxlinkdim={p.state.maxlinkdim()}", file=sys.stderr, flush=True)
             t[0] = now
         return eigsolve_problem(prob, nsweeps=n, nsites=2, sweep_printer=progress,
+                                updater_kwargs=dict(precision=precision),
                                 inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=maxdim)))
 
     if sched:
@@ -291,15 +297,28 @@ def qn_workload(args):
     t_prof = time.perf_counter() - t0
     stats = ctx.profile_stats()
     ctx.profile(False)
+    plain = None
+    if args.precision != "fp64":
+        ctx.sync()
+        t0 = time.perf_counter()
+        _, psi64 = sweeps(1, args.maxdim, "fp64")
+        ctx.sync()
+        t64 = time.perf_counter() - t0
+        E64 = ns.expect_mpo(psi64, H)
+        plain = {"sweeps_per_s": round(1.0 / t64, 6), "speedup_of_form": round(t64 * args.steps / t_timed, 3),
+                 "E_fp64_after_sweep": E64, "rel_dE": abs(E_psi - E64) / abs(E64)}
     ctx.close()
     total_ms = 1e3 * t_timed
-    roof, share = _roofline(stats, "block", 1e3 * t_prof)
+    roof, share = _roofline(stats, "block", 1e3 * t_prof, PEAK_TFLOPS[args.precision])
     config = {"workload": f"2-site U(1) block-sparse DMRG, spin-1 Heisenberg chain N={N}, d=3, MPO D=5, total Sz=0 "
                           f"(Neel start), maxdim {args.maxdim}, cutoff {args.cutoff:g}, Lanczos K=3",
-              "precision": "fp64", "regions_per_sweep": 2 * (N - 1), "maxlinkdim_timed": psi.maxlinkdim(),
+              "precision": args.precision, "regions_per_sweep": 2 * (N - 1), "maxlinkdim_timed": psi.maxlinkdim(),
               "max_sectors_per_link": nsec, "energy": E_psi, "parallelism": "single GPU"}
-    return _common(args, "QN DMRG sweeps/s (spin-1 N=64, Sz=0)", args.steps / t_timed, "sweeps/s", total_ms,
-                   args.steps, "f64", config, roof, share)
+    line = _common(args, "QN DMRG sweeps/s (spin-1 N=64, Sz=0)", args.steps / t_timed, "sweeps/s", total_ms,
+                   args.steps, {"fp64": "f64", "fp32": "f32+f64"}[args.precision], config, roof, share)
+    if plain:
+        line["fp64_plain"] = plain
+    return line
 
 
 def tree_tdvp_workload(args):

@@ -241,12 +241,14 @@ int nsmi_netop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* o
  * over copy_items rows (src_buf, src_off, src_ld, dst_buf, dst_off, dst_ld, rows,
  * cols).  Every block is checked against its buffer here.  With x_elems ==
  * y_elems the handle is an effective Hamiltonian for nsmi_heff_apply/_rayleigh,
- * nsmi_lanczos_eig and nsmi_krylov_expm; any program runs with nsmi_bsop_run. */
+ * nsmi_lanczos_eig and nsmi_krylov_expm; any program runs with nsmi_bsop_run.
+ * precision NSMI_PREC_FP32 runs the program on FP32 copies of the operands and
+ * FP32 scratch (FP32 MFMA GEMMs), x and y converted per run; NSMI_PREC_FP64 as is. */
 int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* operands, int n_scratch,
                      const int64_t* scratch_elems, int64_t x_elems, int64_t y_elems, int n_steps,
                      const int64_t* steps, const int64_t* gemm_items, const int64_t* segs, const double* seg_alpha,
                      const int64_t* lin_items, const int64_t* terms, const double* term_coeff,
-                     const int64_t* copy_items, nsmi_heff** out);
+                     const int64_t* copy_items, int precision, nsmi_heff** out);
 int nsmi_bsop_run(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y);
 
 /* Truncated factorisation of a block-diagonal matrix (ITensors factorize of a QN

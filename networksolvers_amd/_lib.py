@@ -98,7 +98,7 @@ SIGNATURES = {
     "nsmi_bsop_create": (C.c_int, [_VP, C.c_int, C.POINTER(_VP), C.c_int, C.POINTER(C.c_int64), C.c_int64,
                                    C.c_int64, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                    C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64),
-                                   C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), _PP]),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int, _PP]),
     "nsmi_bsop_run": (C.c_int, [_VP, _VP, _VP, _VP]),
     "nsmi_factorize_sectors": (C.c_int, [_VP, _VP, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                          C.POINTER(C.c_int64), C.c_int, C.POINTER(Trunc), _PP, _PP,

@@ -23,8 +23,23 @@ namespace {
 
 constexpr int kMaxBufs = 32;
 
+template <typename T>
 struct BufTable {
-  double* p[kMaxBufs];
+  T* p[kMaxBufs];
+};
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct BsT;
+template <> struct BsT<double> {
+  typedef double __attribute__((ext_vector_type(4))) acc;
+  __device__ static acc mfma(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kl, int r) { return kl + 4 * r; }  // f64 16x16x4 output rows interleave
+};
+template <> struct BsT<float> {
+  typedef f4 acc;
+  __device__ static acc mfma(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int kl, int r) { return 4 * kl + r; }
 };
 
 struct GItem {  // one output block
@@ -59,32 +74,32 @@ struct Chunk {  // 256 consecutive elements of one lincomb / copy item
   int64_t start;
 };
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-template <int BM, int BN, bool TA, bool TB>
-__global__ __launch_bounds__(256) void k_bs_gemm(BufTable bt, const GItem* __restrict__ items,
+template <typename T, int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_bs_gemm(BufTable<T> bt, const GItem* __restrict__ items,
                                                  const GSeg* __restrict__ segs, const GTile* __restrict__ tiles) {
+  typedef typename BsT<T>::acc Acc;
   constexpr int BK = 16, PAD = 4;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
-  __shared__ double sA[BK][BM + PAD];
-  __shared__ double sB[BK][BN + PAD];
+  __shared__ T sA[BK][BM + PAD];
+  __shared__ T sB[BK][BN + PAD];
   const GTile t = tiles[blockIdx.x];
   const GItem it = items[t.item];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t m0 = int64_t(t.tm) * BM, n0 = int64_t(t.tn) * BN;
   const int kl = lane >> 4, ml = lane & 15;
-  d4 acc[FM][FN];
+  Acc acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+    for (int j = 0; j < FN; ++j) acc[i][j] = Acc{0, 0, 0, 0};
   for (int s = 0; s < it.seg_count; ++s) {
     const GSeg sg = segs[it.seg_first + s];
-    const double* __restrict__ A = bt.p[sg.a_buf] + sg.a_off;
-    const double* __restrict__ B = bt.p[sg.b_buf] + sg.b_off;
+    const T* __restrict__ A = bt.p[sg.a_buf] + sg.a_off;
+    const T* __restrict__ B = bt.p[sg.b_buf] + sg.b_off;
+    const T alpha = T(sg.alpha);
     for (int64_t k0 = 0; k0 < sg.K; k0 += BK) {
 #pragma unroll
       for (int j = 0; j < EA; ++j) {
@@ -92,8 +107,8 @@ __global__ __launch_bounds__(256) void k_bs_gemm(BufTable bt, const GItem* __res
         const int mm = TA ? (e % BM) : (e / BK);
         const int kk = TA ? (e / BM) : (e % BK);
         const int64_t gm = m0 + mm, gk = k0 + kk;
-        double v = 0.0;
-        if (gm < it.M && gk < sg.K) v = sg.alpha * (TA ? A[gk * sg.lda + gm] : A[gm * sg.lda + gk]);
+        T v = T(0);
+        if (gm < it.M && gk < sg.K) v = alpha * (TA ? A[gk * sg.lda + gm] : A[gm * sg.lda + gk]);
         sA[kk][mm] = v;
       }
 #pragma unroll
@@ -102,14 +117,14 @@ __global__ __launch_bounds__(256) void k_bs_gemm(BufTable bt, const GItem* __res
         const int nn = TB ? (e / BK) : (e % BN);
         const int kk = TB ? (e % BK) : (e / BN);
         const int64_t gn = n0 + nn, gk = k0 + kk;
-        double v = 0.0;
+        T v = T(0);
         if (gn < it.N && gk < sg.K) v = TB ? B[gn * sg.ldb + gk] : B[gk * sg.ldb + gn];
         sB[kk][nn] = v;
       }
       __syncthreads();
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
-        double af[FM], bf[FN];
+        T af[FM], bf[FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i) af[i] = sA[ks * 4 + kl][wm * WTM + i * 16 + ml];
 #pragma unroll
@@ -118,39 +133,41 @@ __global__ __launch_bounds__(256) void k_bs_gemm(BufTable bt, const GItem* __res
         for (int i = 0; i < FM; ++i)
 #pragma unroll
           for (int j = 0; j < FN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = BsT<T>::mfma(af[i], bf[j], acc[i][j]);
       }
       __syncthreads();
     }
   }
-  double* __restrict__ C = bt.p[it.c_buf] + it.c_off;
+  T* __restrict__ C = bt.p[it.c_buf] + it.c_off;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gm = m0 + wm * WTM + i * 16 + kl + 4 * r;  // f64 16x16x4 output rows interleave
+        const int64_t gm = m0 + wm * WTM + i * 16 + BsT<T>::row(kl, r);
         const int64_t gn = n0 + wn * WTN + j * 16 + ml;
         if (gm < it.M && gn < it.N) C[gm * it.ldc + gn] = acc[i][j][r];
       }
 }
 
-__global__ __launch_bounds__(256) void k_bs_lincomb(BufTable bt, const LItem* __restrict__ items,
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_lincomb(BufTable<T> bt, const LItem* __restrict__ items,
                                                     const LTerm* __restrict__ terms, const Chunk* __restrict__ chunks) {
   const Chunk c = chunks[blockIdx.x];
   const LItem it = items[c.item];
   const int64_t i = c.start + threadIdx.x;
   if (i >= it.n) return;
-  double acc = 0.0;
+  T acc = T(0);
   for (int t = 0; t < it.term_count; ++t) {
     const LTerm tm = terms[it.term_first + t];
-    acc += tm.coeff * bt.p[tm.src_buf][tm.src_off + i];
+    acc += T(tm.coeff) * bt.p[tm.src_buf][tm.src_off + i];
   }
   bt.p[it.out_buf][it.out_off + i] = acc;
 }
 
-__global__ __launch_bounds__(256) void k_bs_copy(BufTable bt, const CItem* __restrict__ items,
+template <typename T>
+__global__ __launch_bounds__(256) void k_bs_copy(BufTable<T> bt, const CItem* __restrict__ items,
                                                  const Chunk* __restrict__ chunks) {
   const Chunk c = chunks[blockIdx.x];
   const CItem it = items[c.item];
@@ -177,7 +194,11 @@ struct BsStep {
 
 struct BsProgram {
   int n_operands = 0;
+  bool lo = false;               // FP32 form: FP32 operand copies, scratch, x and y
   std::vector<const nsmi_tensor*> operands;
+  std::vector<void*> op32;       // FP32 copies of the operands (FP32 form)
+  void* xlo = nullptr;
+  void* ylo = nullptr;
   std::vector<void*> scratch;
   std::vector<int64_t> sizes;  // elements per buffer id
   std::vector<BsStep> steps;
@@ -193,6 +214,9 @@ void bs_release(nsmi_ctx* ctx, BsProgram* p) {
     scratch_release(ctx, s.sched);
   }
   for (void* s : p->scratch) scratch_release(ctx, s);
+  for (void* s : p->op32) scratch_release(ctx, s);
+  scratch_release(ctx, p->xlo);
+  scratch_release(ctx, p->ylo);
   delete p;
 }
 
@@ -216,50 +240,66 @@ void check_range(const BsProgram* p, int buf, int64_t off, int64_t rows, int64_t
   require(last <= p->sizes[buf], std::string("bsop: block outside its buffer in ") + what);
 }
 
-template <int BM, int BN>
-void launch_gemm(nsmi_ctx* ctx, const BufTable& bt, const BsStep& s) {
+template <typename T, int BM, int BN>
+void launch_gemm(nsmi_ctx* ctx, const BufTable<T>& bt, const BsStep& s) {
   const dim3 grid(unsigned(s.nsched)), block(256);
   const GItem* it = static_cast<const GItem*>(s.items);
   const GSeg* sg = static_cast<const GSeg*>(s.aux);
   const GTile* tl = static_cast<const GTile*>(s.sched);
   if (!s.ta && !s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<BM, BN, false, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, false, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
   else if (!s.ta && s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<BM, BN, false, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, false, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
   else if (s.ta && !s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<BM, BN, true, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, true, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
   else
-    hipLaunchKernelGGL((k_bs_gemm<BM, BN, true, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, true, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
   NSMI_LAUNCH_CHECK();
+}
+
+template <typename T>
+void run_steps(nsmi_ctx* ctx, const BsProgram* p, const BufTable<T>& bt) {
+  for (const BsStep& s : p->steps) {
+    if (s.nsched == 0) continue;
+    if (s.kind == 0) {
+      if (s.tile == 64)
+        launch_gemm<T, 64, 64>(ctx, bt, s);
+      else
+        launch_gemm<T, 32, 32>(ctx, bt, s);
+    } else if (s.kind == 1) {
+      hipLaunchKernelGGL(k_bs_lincomb<T>, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
+                         static_cast<const LItem*>(s.items), static_cast<const LTerm*>(s.aux),
+                         static_cast<const Chunk*>(s.sched));
+      NSMI_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(k_bs_copy<T>, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
+                         static_cast<const CItem*>(s.items), static_cast<const Chunk*>(s.sched));
+      NSMI_LAUNCH_CHECK();
+    }
+  }
 }
 
 }  // namespace
 
 void bs_run(nsmi_ctx* ctx, BsProgram* p, const double* x, double* y) {
-  BufTable bt{};
+  ProfScope ps(ctx, NSMI_K_BLOCK, p->flops, 0.0);
+  if (p->lo) {
+    BufTable<float> bt{};
+    if (x && p->x_elems) vec_to_f32(ctx, static_cast<float*>(p->xlo), x, p->x_elems);
+    bt.p[0] = static_cast<float*>(p->xlo);
+    bt.p[1] = static_cast<float*>(p->ylo);
+    for (int i = 0; i < p->n_operands; ++i) bt.p[2 + i] = static_cast<float*>(p->op32[i]);
+    for (size_t i = 0; i < p->scratch.size(); ++i) bt.p[2 + p->n_operands + i] = static_cast<float*>(p->scratch[i]);
+    run_steps<float>(ctx, p, bt);
+    if (y && p->y_elems) vec_f32_to_f64(ctx, y, static_cast<const float*>(p->ylo), p->y_elems);
+    return;
+  }
+  BufTable<double> bt{};
   bt.p[0] = const_cast<double*>(x);
   bt.p[1] = y;
   for (int i = 0; i < p->n_operands; ++i) bt.p[2 + i] = static_cast<double*>(p->operands[i]->data);
   for (size_t i = 0; i < p->scratch.size(); ++i) bt.p[2 + p->n_operands + i] = static_cast<double*>(p->scratch[i]);
-  ProfScope ps(ctx, NSMI_K_BLOCK, p->flops, 0.0);
-  for (const BsStep& s : p->steps) {
-    if (s.nsched == 0) continue;
-    if (s.kind == 0) {
-      if (s.tile == 64)
-        launch_gemm<64, 64>(ctx, bt, s);
-      else
-        launch_gemm<32, 32>(ctx, bt, s);
-    } else if (s.kind == 1) {
-      hipLaunchKernelGGL(k_bs_lincomb, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
-                         static_cast<const LItem*>(s.items), static_cast<const LTerm*>(s.aux),
-                         static_cast<const Chunk*>(s.sched));
-      NSMI_LAUNCH_CHECK();
-    } else {
-      hipLaunchKernelGGL(k_bs_copy, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
-                         static_cast<const CItem*>(s.items), static_cast<const Chunk*>(s.sched));
-      NSMI_LAUNCH_CHECK();
-    }
-  }
+  run_steps<double>(ctx, p, bt);
 }
 
 int64_t bs_x_elems(const BsProgram* p) { return p->x_elems; }
@@ -287,16 +327,19 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
                      const int64_t* scratch_elems, int64_t x_elems, int64_t y_elems, int n_steps,
                      const int64_t* steps, const int64_t* gemm_items, const int64_t* segs, const double* seg_alpha,
                      const int64_t* lin_items, const int64_t* terms, const double* term_coeff,
-                     const int64_t* copy_items, nsmi_heff** out) {
+                     const int64_t* copy_items, int precision, nsmi_heff** out) {
   if (!ctx || !out) return NSMI_ERR_INVALID;
   BsProgram* p = nullptr;
   API_BEGIN
   *out = nullptr;
+  require(precision == NSMI_PREC_FP64 || precision == NSMI_PREC_FP32, "bsop: precision must be FP64 or FP32");
   require(n_operands >= 0 && n_scratch >= 0 && 2 + n_operands + n_scratch <= kMaxBufs,
           "bsop: at most 30 operand and scratch buffers");
   require(x_elems >= 0 && y_elems >= 0 && n_steps >= 0 && (n_steps == 0 || steps), "bsop: bad program header");
   p = new BsProgram();
   p->n_operands = n_operands;
+  p->lo = precision == NSMI_PREC_FP32;
+  const size_t elt = p->lo ? sizeof(float) : sizeof(double);
   p->x_elems = x_elems;
   p->y_elems = y_elems;
   p->sizes.push_back(x_elems);
@@ -305,11 +348,20 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
     require(operands[i] && operands[i]->dtype == NSMI_F64, "bsop: operands must be FP64 tensors");
     p->operands.push_back(operands[i]);
     p->sizes.push_back(operands[i]->numel);
+    if (p->lo) {
+      p->op32.push_back(scratch(ctx, sizeof(float) * size_t(std::max<int64_t>(operands[i]->numel, 1))));
+      vec_to_f32(ctx, static_cast<float*>(p->op32.back()), static_cast<const double*>(operands[i]->data),
+                 operands[i]->numel);
+    }
   }
   for (int i = 0; i < n_scratch; ++i) {
     require(scratch_elems[i] >= 0, "bsop: negative scratch size");
-    p->scratch.push_back(scratch(ctx, sizeof(double) * size_t(std::max<int64_t>(scratch_elems[i], 1))));
+    p->scratch.push_back(scratch(ctx, elt * size_t(std::max<int64_t>(scratch_elems[i], 1))));
     p->sizes.push_back(scratch_elems[i]);
+  }
+  if (p->lo) {
+    p->xlo = scratch(ctx, sizeof(float) * size_t(std::max<int64_t>(x_elems, 1)));
+    p->ylo = scratch(ctx, sizeof(float) * size_t(std::max<int64_t>(y_elems, 1)));
   }
   for (int si = 0; si < n_steps; ++si) {
     const int64_t* st = steps + 6 * si;  // kind, transA, transB, first, count, aux

@@ -195,7 +195,7 @@ class _Program:
         """Buffer id of scratch buffer j (valid once all operands are registered)."""
         return 2 + len(self.operands) + j
 
-    def build(self, ctx: Context, x_elems, y_elems):
+    def build(self, ctx: Context, x_elems, y_elems, precision="fp64"):
         def arr(rows, w, dt=C.c_int64):
             flat = [v for r in rows for v in r] if rows else [0] * w
             return (dt * max(len(flat), 1))(*flat)
@@ -207,18 +207,21 @@ class _Program:
             ctx.h, len(self.operands), ops, len(self.scratch), scr, int(x_elems), int(y_elems), len(self.steps),
             arr(self.steps, 6), arr(self.gitems, 7), arr(self.segs, 7),
             (C.c_double * max(len(self.alpha), 1))(*self.alpha), arr(self.litems, 5), arr(self.terms, 2),
-            (C.c_double * max(len(self.coeff), 1))(*self.coeff), arr(self.citems, 8), C.byref(h)))
-        return BlockProgram(ctx, h, self.operands)
+            (C.c_double * max(len(self.coeff), 1))(*self.coeff), arr(self.citems, 8), kernels._prec(precision),
+            C.byref(h)))
+        return BlockProgram(ctx, h, self.operands, precision)
 
 
 class BlockProgram(kernels._OperatorBase):
-    """A built block program; an effective Hamiltonian when x and y have one size."""
+    """A built block program; an effective Hamiltonian when x and y have one size.
+    ``precision`` "fp32" runs its GEMMs, mixing and copies in FP32 (FP32 MFMA) on
+    FP32 copies of the operands, with x and y converted per run."""
 
-    def __init__(self, ctx, h, keep):
+    def __init__(self, ctx, h, keep, precision="fp64"):
         self.ctx = ctx
         self.h = h
         self._keep = tuple(keep)
-        self.precision = "fp64"
+        self.precision = precision
 
     def run(self, x: DeviceTensor | None, y: DeviceTensor | None):
         self.ctx.check(self.ctx.lib.nsmi_bsop_run(self.ctx.h, self.h, x.h if x is not None else None,
@@ -443,10 +446,10 @@ def theta(A: QNTensor, B: QNTensor, ctx) -> QNTensor:
     return th
 
 
-def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx) -> BlockProgram:
+def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx, precision="fp64") -> BlockProgram:
     """Two-site effective Hamiltonian (operator_map.jl:3-10) on theta's block space:
     T1 = L.theta (GEMM), T3 = (W1 W2)-mix (LINCOMB), y = T3.R^T (GEMM, segments over
-    the right MPO link)."""
+    the right MPO link); in FP64 or the FP32 form."""
     l, s1i, s2i, r = th.indices
     W12 = np.einsum("wmas,mvbt->wstabv", W1, W2)   # [w, s1, s2, s1', s2', v]
     nz = {}
@@ -488,7 +491,7 @@ def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx) -> BlockP
             sg.append((s3b, T3[(ap, a1, a2, v, b)], r.n[b], xR, R.block_off((bp, v, b)), r.n[b], r.n[b], 1.0))
         items.append((1, th.block_off((ap, a1, a2, bp)), r.n[bp], l.n[ap], r.n[bp], sg))
     p.gemm(False, True, items)
-    return p.build(ctx, th.size, th.size)
+    return p.build(ctx, th.size, th.size, precision)
 
 
 def factorize(th: QNTensor, rows_trailing: bool, *, cutoff=0.0, maxdim=None, mindim=1, which_decomp=None, ctx=None):
@@ -647,11 +650,11 @@ def extract(psi: QNMPS, region):
     return theta(psi[lo], psi[hi], psi.ctx)
 
 
-def heff_map(P: QNProjTTN, psi: QNMPS):
+def heff_map(P: QNProjTTN, psi: QNMPS, precision="fp64"):
     lo, hi = min(P.pos), max(P.pos)
     L, R = P.left_right(psi)
     th = QNTensor(psi.ctx, [psi.links[lo], psi.site, psi.site, psi.links[hi + 1]], [1, 1, 1, -1], 0)
-    return heff2(L, P.H.W[lo], P.H.W[hi], R, th, P.H.c[lo], P.H.c[hi + 1], psi.ctx)
+    return heff2(L, P.H.W[lo], P.H.W[hi], R, th, P.H.c[lo], P.H.c[hi + 1], psi.ctx, precision)
 
 
 def insert(psi: QNMPS, region, th: QNTensor, trunc, which_decomp=None):

@@ -230,9 +230,7 @@ def optimal_map(P: ProjTTN, psi: TTN, precision="fp64") -> kernels.HeffMap:
     (environments of the incident edges and the region's MPO tensors; on a tree a
     contraction program over up to four environments)."""
     if isinstance(P, qn.QNProjTTN):
-        if precision != "fp64":
-            raise ValueError("QN regions are served in FP64 only")
-        return qn.heff_map(P, psi)
+        return qn.heff_map(P, psi, precision)
     if isinstance(P, tree.TreeProjTTN):
         region = P.sites()
         return P.netmap(psi, tree.region_labels(psi, region), tree.region_shape(psi, region), precision)

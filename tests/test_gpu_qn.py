@@ -83,6 +83,45 @@ def test_qn_heff2_apply(ctx):
     op.close()
 
 
+def test_qn_heff2_fp32_form(ctx):
+    """The FP32 form of the block program (FP32 operand copies, FP32 MFMA block
+    GEMMs, FP32 mixing) equals the dense contraction to FP32 rounding (2e-5
+    relative), over repeated applies."""
+    from networksolvers_amd import qn
+    site, l, m, r, H = _setup(ctx)
+    L = _rand_qn(ctx, [l, H.links[1], l], [1, -1, -1], 5)
+    R = _rand_qn(ctx, [r, H.links[3], r], [1, -1, -1], 6)
+    th = _rand_qn(ctx, [l, site, site, r], [1, 1, 1, -1], 7)
+    op = qn.heff2(L, H.W[1], H.W[2], R, th, H.c[1], H.c[3], ctx, "fp32")
+    want = chain.heff2_apply(L.to_dense(), H.W[1], H.W[2], R.to_dense(), th.to_dense())
+    for _ in range(2):
+        y = qn.QNTensor(ctx, th.indices, th.dirs, 0, th.keys, op(th.data))
+        assert rel(y.to_dense(), want) < 2e-5
+    op.close()
+
+
+def test_qn_dmrg_fp32_form_energy(ctx):
+    """examples/dmrg.jl with conserve_qns in the FP32 H_eff form: the FP64
+    <psi|H|psi> of the result equals the FP64 form's to 1e-9 relative and the
+    reference's -12.8945601 to 1e-7."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import qn
+    N = 10
+    W = models.heisenberg_mpo(N, d=3)
+    states = [0 if (i + 1) % 2 == 0 else 2 for i in range(N)]
+    site = qn.spin_site_index(3)
+    H = qn.QNMPO(ctx, W, site)
+    trunc = dict(cutoff=1e-12, maxdim=[10, 40, 80, 160, 243])
+    res = {}
+    for prec in ("fp64", "fp32"):
+        prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site), operator=ns.sweep.projected_operator(H))
+        E, psi = ns.sweep.eigsolve_problem(prob, nsweeps=6, nsites=2, updater_kwargs=dict(precision=prec),
+                                           inserter_kwargs=dict(trunc=trunc))
+        res[prec] = ns.expect_mpo(psi, H)
+    assert abs(res["fp32"] - res["fp64"]) < 1e-9 * abs(res["fp64"])
+    assert abs(res["fp32"] - (-12.8945601)) < 1e-7
+
+
 @pytest.mark.parametrize("rows_trailing", [False, True])
 @pytest.mark.parametrize("cutoff,maxdim", [(1e-14, None), (1e-6, 8), (1e-3, None)])
 def test_qn_factorize_vs_oracle(ctx, rows_trailing, cutoff, maxdim):
