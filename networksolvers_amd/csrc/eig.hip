@@ -16,9 +16,8 @@
 // column-major view the diagonal d, the subdiagonal e and v_j(2:) in A(j+2:n, j)
 // -- which rocsolver_dstedc and rocsolver_dormtr consume.
 //
-// Distribution: workgroup g owns the rows i = g (mod G); row i sits in the
-// registers of one wave (slot s of wave w holds row (4s + w) G + g), lane l
-// holding the columns l + 64c.  Step j:
+// Distribution: workgroup g owns the rows i = g (mod G); row i sits in registers
+// (of one wave, or of one 16-lane group; see the two kernels).  Step j:
 //   1. every workgroup forms v from the column x_j (kept in LDS by all of them);
 //   2. each wave computes p_i = tau v.row_i for its rows i > j and publishes it;
 //      the owner of row j+1 publishes that row (= column j+1, pre-update) as Y;
@@ -117,6 +116,166 @@ __device__ inline bool grid_barrier(unsigned* bar, unsigned target, unsigned* fa
   return *s_ok != 0;
 }
 
+// Sum over each 16-lane row of the wave (quad permutes, half-row and row mirror)
+// for R values at once; every lane of a row gets its row's total.
+template <int R>
+__device__ inline void row16_totals(double (&a)[R]) {
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0xB1, 0xF>(a[s]);  // quad_perm [1,0,3,2]
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x4E, 0xF>(a[s]);  // quad_perm [2,3,0,1]
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x141, 0xF>(a[s]);  // row_half_mirror
+#pragma unroll
+  for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x140, 0xF>(a[s]);  // row_mirror
+}
+
+// Row-group layout (the form for one workgroup, n <= 128, where a wave would
+// otherwise own up to 32 rows and their 6-stage reductions): the 256 threads form
+// 16 groups of 16 lanes (DPP rows); group rg of
+// workgroup g owns the matrix rows (r 16 + rg) G + g, r < RS, in registers, its
+// lane l holding columns l + 16c, c < CC (NP = 16 CC >= n).  So a row's dot
+// product with v is a 16-lane reduction.  v and w live in LDS, double-buffered by
+// the parity of j (a fast wave may write step j+1's v while a slow one still
+// updates with step j's); x_j lives in LDS too.  SINGLE: one workgroup (G == 1),
+// the p / Y exchange in LDS and the barrier a workgroup barrier; otherwise xch
+// holds 4 n doubles of global exchange (P[2][n], Y[2][n]); bar, fail: zero on entry.
+template <int CC, int RS, bool SINGLE>
+__global__ __launch_bounds__(kTriThreads) void k_sytrd_grp(double* __restrict__ A, int n, double* __restrict__ d,
+                                                           double* __restrict__ e, double* __restrict__ tau,
+                                                           double* __restrict__ xch, unsigned* bar,
+                                                           unsigned* fail) {
+  constexpr int NP = 16 * CC;  // padded vector length (a multiple of 64)
+  constexpr int NQ = NP / 64;  // entries per lane of a wave-wide vector pass
+  __shared__ double vs[2][NP], ws[2][NP], xs[NP];
+  __shared__ double xl[SINGLE ? 4 * NP : 1];
+  __shared__ int s_ok;
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rg = tid >> 4, l16 = tid & 15;
+  double* const xbase = SINGLE ? xl : xch;
+  const int xn = SINGLE ? NP : n;  // stride of the exchange halves
+
+  double row[RS][CC];
+  int rid[RS];
+#pragma unroll
+  for (int r = 0; r < RS; ++r) {
+    const int i = (r * 16 + rg) * G + g;
+    rid[r] = i < n ? i : -1;
+#pragma unroll
+    for (int c = 0; c < CC; ++c) {
+      const int k = l16 + 16 * c;
+      row[r][c] = (i < n && k < n) ? A[size_t(i) * n + k] : 0.0;
+    }
+  }
+  // x_0 = column 0 = row 0 (symmetric); nobody writes A before the first barrier
+  for (int k = tid; k < NP; k += kTriThreads) xs[k] = (k >= 1 && k < n) ? A[k] : 0.0;
+  if (g == 0 && tid == 0) d[0] = A[0];
+  __syncthreads();
+
+  unsigned nbar = 0;
+  for (int j = 0; j + 1 < n; ++j) {
+    double* const vb = vs[j & 1];
+    double* const wb = ws[j & 1];
+    double* const P = xbase + (j & 1) * xn;
+    double* const Y = xbase + 2 * xn + (j & 1) * xn;
+    // 1. reflector from x = xs[j+1 ..] (every wave reduces the whole vector)
+    double part = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int k = lane + 64 * q;
+      if (k >= j + 2) part += xs[k] * xs[k];
+    }
+    const double sigma2 = wave_total(part);
+    const double alpha = xs[j + 1];
+    double t = 0.0, beta = alpha, scal = 0.0;
+    if (sigma2 != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma2), alpha);
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    for (int k = tid; k < NP; k += kTriThreads) vb[k] = (k == j + 1) ? 1.0 : (k >= j + 2 ? xs[k] * scal : 0.0);
+    __syncthreads();
+    // 2. p_i = tau v . row_i for own rows i > j; the owner of row j+1 publishes it as Y
+    double acc[RS];
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      acc[r] = 0.0;
+#pragma unroll
+      for (int c = 0; c < CC; ++c)
+        if (16 * c + 15 > j) acc[r] += row[r][c] * vb[l16 + 16 * c];
+    }
+    row16_totals<RS>(acc);
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      if (rid[r] <= j) continue;  // uniform per 16-lane group (rid < 0: empty slot)
+      if (l16 == 0) P[rid[r]] = t * acc[r];
+      if (rid[r] == j + 1) {
+#pragma unroll
+        for (int c = 0; c < CC; ++c) {
+          const int k = l16 + 16 * c;
+          if (k > j && k < n) Y[k] = row[r][c];
+        }
+      }
+    }
+    // 3.
+    if (SINGLE) {
+      __syncthreads();
+    } else if (!grid_barrier(bar, unsigned(G) * ++nbar, fail, &s_ok)) {
+      return;
+    }
+    if (g == 0) {
+      // LAPACK layout of column j (column-major view = row j of the buffer)
+      for (int k = tid; k < NP; k += kTriThreads)
+        if (k >= j + 2 && k < n) A[size_t(j) * n + k] = vb[k];
+      if (tid == 0) {
+        e[j] = beta;
+        tau[j] = t;
+        A[size_t(j) * n + j + 1] = beta;
+      }
+    }
+    // 4. w = p - (tau/2)(p.v) v (every wave forms p.v over the whole vector), the
+    // next column x_{j+1} and d_{j+1}; wave w writes the entries lane + 64q, q = w mod 4
+    double pq[NQ];
+    part = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int k = lane + 64 * q;
+      pq[q] = (k > j && k < n) ? P[k] : 0.0;
+      part += pq[q] * vb[k];
+    }
+    const double cc = 0.5 * t * wave_total(part);
+    const double w1 = P[j + 1] - cc;  // w_{j+1}, as v_{j+1} = 1
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if ((q & 3) != wave) continue;  // uniform per wave
+      const int k = lane + 64 * q;
+      const double wk = pq[q] - cc * vb[k];
+      wb[k] = wk;
+      xs[k] = (k >= j + 2 && k < n) ? Y[k] - vb[k] * w1 - wk : 0.0;
+    }
+    if (g == 0 && tid == 0) {
+      const double dn = Y[j + 1] - 2.0 * w1;
+      d[j + 1] = dn;
+      A[size_t(j + 1) * n + j + 1] = dn;
+    }
+    __syncthreads();
+    // 5. rank-2 update of the own rows
+#pragma unroll
+    for (int r = 0; r < RS; ++r) {
+      if (rid[r] <= j) continue;
+      const double vi = vb[rid[r]], wi = wb[rid[r]];
+#pragma unroll
+      for (int c = 0; c < CC; ++c)
+        if (16 * c + 15 > j) {
+          const int k = l16 + 16 * c;
+          row[r][c] -= vi * wb[k] + wi * vb[k];
+        }
+    }
+  }
+}
+
+// Wave-owned rows (the form for several workgroups): slot s of wave w holds row
+// (4s + w) G + g, lane l holding the columns l + 64c.
 // CL: 64-column slots per lane (n <= 64 CL); RW: rows per wave (G = ceil(n / (4 RW))).
 // SINGLE: one workgroup (G == 1), the p / Y exchange in LDS and the barrier a
 // workgroup barrier.  xch: 4 n doubles of global exchange (P[2][n], Y[2][n]) when
@@ -127,7 +286,7 @@ __device__ inline bool grid_barrier(unsigned* bar, unsigned target, unsigned* fa
 // for the broadcast reads v_i, w_i of its rows; so the only synchronisation of a
 // step is the barrier between publishing p / Y and reading them.
 template <int CL, int RW, bool SINGLE>
-__global__ __launch_bounds__(kTriThreads) void k_sytrd_coop(double* __restrict__ A, int n, double* __restrict__ d,
+__global__ __launch_bounds__(kTriThreads) void k_sytrd_wave(double* __restrict__ A, int n, double* __restrict__ d,
                                                             double* __restrict__ e, double* __restrict__ tau,
                                                             double* __restrict__ xch, unsigned* bar,
                                                             unsigned* fail) {
@@ -268,13 +427,15 @@ struct TriPlan {
   int rows_per_group;
 };
 
-// the smallest instantiation whose padded width covers n
+// the smallest instantiation whose padded width covers n: the row-group form on
+// one workgroup, the wave-owned form on several (measured faster there: it needs
+// no workgroup barrier besides the grid barrier)
 TriPlan tri_plan(int n) {
-  if (n <= 64) return {reinterpret_cast<const void*>(k_sytrd_coop<1, 16, true>), 64};
-  if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_coop<2, 32, true>), 128};
-  if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_coop<4, 16, false>), 64};
-  if (n <= 512) return {reinterpret_cast<const void*>(k_sytrd_coop<8, 8, false>), 32};
-  return {reinterpret_cast<const void*>(k_sytrd_coop<16, 4, false>), 16};
+  if (n <= 64) return {reinterpret_cast<const void*>(k_sytrd_grp<4, 4, true>), 64};
+  if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_grp<8, 8, true>), 128};
+  if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_wave<4, 16, false>), 64};
+  if (n <= 512) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 8, false>), 32};
+  return {reinterpret_cast<const void*>(k_sytrd_wave<16, 4, false>), 16};
 }
 
 // ---- batched Jacobi eigensolver for small Gram matrices ---------------------
