@@ -30,7 +30,9 @@
 // and before barrier j+1, when every workgroup has finished reading step j-1's.
 // Barrier spins are bounded: a workgroup that waits too long raises a flag and
 // leaves, every other one leaves at its next barrier once it sees the flag, so
-// the grid always drains; the host then reports the failure.
+// the grid always drains; the host then reports the failure.  fail[0] is that
+// flag; fail[1] is raised when an entry of d or e is NaN or Inf (no barrier or
+// loop bound depends on the values, so such input runs the fixed step count).
 #include "internal.h"
 
 namespace nsmi {
@@ -170,6 +172,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_grp(double* __restrict__ 
   // x_0 = column 0 = row 0 (symmetric); nobody writes A before the first barrier
   for (int k = tid; k < NP; k += kTriThreads) xs[k] = (k >= 1 && k < n) ? A[k] : 0.0;
   if (g == 0 && tid == 0) d[0] = A[0];
+  bool finite = isfinite(A[0]);  // of d and e, kept by the thread that writes them
   __syncthreads();
 
   unsigned nbar = 0;
@@ -231,6 +234,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_grp(double* __restrict__ 
         e[j] = beta;
         tau[j] = t;
         A[size_t(j) * n + j + 1] = beta;
+        finite = finite && isfinite(beta);
       }
     }
     // 4. w = p - (tau/2)(p.v) v (every wave forms p.v over the whole vector), the
@@ -257,6 +261,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_grp(double* __restrict__ 
       const double dn = Y[j + 1] - 2.0 * w1;
       d[j + 1] = dn;
       A[size_t(j + 1) * n + j + 1] = dn;
+      finite = finite && isfinite(dn);
     }
     __syncthreads();
     // 5. rank-2 update of the own rows
@@ -272,6 +277,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_grp(double* __restrict__ 
         }
     }
   }
+  if (g == 0 && tid == 0 && !finite) fail[1] = 1u;
 }
 
 // Wave-owned rows (the form for several workgroups): slot s of wave w holds row
@@ -321,6 +327,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_wave(double* __restrict__
   }
   double alpha = n > 1 ? A[1] : 0.0;
   if (g == 0 && tid == 0) d[0] = A[0];
+  bool finite = isfinite(A[0]);  // of d and e, kept by the thread that writes them
 
   unsigned nbar = 0;
   for (int j = 0; j + 1 < n; ++j) {
@@ -384,6 +391,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_wave(double* __restrict__
         e[j] = beta;
         tau[j] = t;
         A[size_t(j) * n + j + 1] = beta;
+        finite = finite && isfinite(beta);
       }
     }
     // 4. w = p - (tau/2)(p.v) v, the next column and d_{j+1}
@@ -408,6 +416,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_wave(double* __restrict__
       const double dn = Y[j + 1] - 2.0 * w1;
       d[j + 1] = dn;
       A[size_t(j + 1) * n + j + 1] = dn;
+      finite = finite && isfinite(dn);
     }
     wave_sync_lds();
     alpha = (j + 2 < n) ? Y[j + 2] - vme[j + 2] * w1 - wme[j + 2] : 0.0;
@@ -420,6 +429,7 @@ __global__ __launch_bounds__(kTriThreads) void k_sytrd_wave(double* __restrict__
         if (64 * c + 63 > j) row[s][c] -= vi * ww[c] + wi * vv[c];
     }
   }
+  if (g == 0 && tid == 0 && !finite) fail[1] = 1u;
 }
 
 struct TriPlan {
@@ -601,10 +611,12 @@ bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* t
       NSMI_HIP(hipLaunchKernel(p.fn, dim3(unsigned(G)), dim3(kTriThreads), args, 0, ctx->stream));
     }
   }
-  unsigned f = 0;
-  NSMI_HIP(hipMemcpyAsync(&f, failp, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  unsigned f[2] = {0u, 0u};
+  NSMI_HIP(hipMemcpyAsync(f, failp, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  if (f) fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
+  if (f[0]) fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
+  // the tridiagonal solver is never handed NaN or Inf
+  if (f[1]) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite entries in the matrix to be factorized");
   return true;
 }
 

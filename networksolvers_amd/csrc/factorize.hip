@@ -296,6 +296,7 @@ void fact_gram(nsmi_ctx* ctx, FactCore& f) {
       check_info(ctx, "rocsolver_dstedc");
     } else {
       f.tau.reset();
+      require_finite(ctx, f.rho->as<double>(), iso * iso, "factorize");
       {
         ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
         check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
@@ -319,6 +320,7 @@ void fact_gram(nsmi_ctx* ctx, FactCore& f) {
     f.ub.reset(new Scratch(ctx, sizeof(double) * size_t(nc) * full));
     f.vb.reset(new Scratch(ctx, sizeof(double) * size_t(full) * m));
     Scratch e(ctx, sizeof(double) * size_t(std::max<int64_t>(full, 2)));
+    require_finite(ctx, f.M, m * nc, "factorize");
     NSMI_HIP(hipMemcpyAsync(a.p, f.M, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
     {
       ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);

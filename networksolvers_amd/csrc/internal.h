@@ -206,6 +206,8 @@ void vec_f32_to_f64(nsmi_ctx* ctx, double* y, const float* x, int64_t n);
 void vec_split_bf16(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const double* x, int64_t n);
 void vec_split_bf16_f32(nsmi_ctx* ctx, uint16_t* hi, uint16_t* lo, const float* x, int64_t n);
 void vec_check_finite(nsmi_ctx* ctx, const double* x, int64_t n, int slot);
+// fails with NSMI_ERR_NOT_FINITE when x holds a NaN or Inf (uses slot 0; synchronises)
+void require_finite(nsmi_ctx* ctx, const double* x, int64_t n, const char* what);
 // complex (interleaved) vectors, n complex elements
 void vec_zscale(nsmi_ctx* ctx, double* x, int64_t n, double ar, double ai);
 void vec_zaxpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double ar, double ai);

@@ -64,6 +64,13 @@ struct KOps {
 
 const double kEps = 2.220446049250313e-16;
 
+// NaN or Inf in the operator or the vector shows up in the recurrence
+// coefficients; the projected problem is never formed from them.
+void require_finite_coefficients(double alpha, double beta) {
+  if (!std::isfinite(alpha) || !std::isfinite(beta))
+    fail(NSMI_ERR_NOT_FINITE, "krylov: non-finite recurrence coefficient (NaN or Inf in the operator or the vector)");
+}
+
 // Lanczos basis growth shared by both solvers: given V[0..k-1] and the residual r
 // with norm beta (r orthogonal to V up to rounding), appends v_k = r / beta, applies
 // H, and returns the new (alpha_k, beta_k) with r the new residual.  Three-term
@@ -93,6 +100,7 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
     a += cl.real();
     b = ops.norm(r);
   }
+  require_finite_coefficients(a, b);
   *alpha_out = a;
   *beta_out = b;
 }
@@ -115,6 +123,7 @@ void lanczos_start(const KOps& ops, nsmi_heff* op, const double* x0, double beta
     ops.axpy(r, V, -d);
     beta = ops.norm(r);
   }
+  require_finite_coefficients(alpha, beta);
   *alpha_out = alpha;
   *beta_out = beta;
 }
@@ -175,6 +184,7 @@ int nsmi_lanczos_eig(nsmi_ctx* ctx, nsmi_heff* op, nsmi_tensor* x, const nsmi_la
   double* r = V + nd * K;
   double* xd = static_cast<double*>(x->data);
   const double beta0 = ops.norm(xd);
+  if (!std::isfinite(beta0)) fail(NSMI_ERR_NOT_FINITE, "lanczos: the initial vector has non-finite entries");
   require(beta0 > 0.0, "lanczos: initial vector should not have norm zero");
   double alpha, beta;
   lanczos_start(ops, op, xd, beta0, V, r, eta, &alpha, &beta);
@@ -251,6 +261,7 @@ int nsmi_krylov_expm(nsmi_ctx* ctx, nsmi_heff* op, double tau_re, double tau_im,
       if (substeps >= maxiter) fail(NSMI_ERR_SOLVER, "expm: exponentiate did not converge within maxiter sub-steps");
       ++substeps;
       const double beta0 = ops.norm(xd);
+      if (!std::isfinite(beta0)) fail(NSMI_ERR_NOT_FINITE, "expm: the vector has non-finite entries");
       if (beta0 == 0.0) break;
       double alpha, beta;
       lanczos_start(ops, op, xd, beta0, V, r, eta, &alpha, &beta);

@@ -334,4 +334,11 @@ void vec_check_finite(nsmi_ctx* ctx, const double* x, int64_t n, int slot) {
   NSMI_LAUNCH_CHECK();
 }
 
+void require_finite(nsmi_ctx* ctx, const double* x, int64_t n, const char* what) {
+  vec_check_finite(ctx, x, n, 0);
+  double count = 0.0;
+  read_slots(ctx, 1, &count);
+  if (count != 0.0) fail(NSMI_ERR_NOT_FINITE, std::string(what) + ": non-finite entries in the input");
+}
+
 }  // namespace nsmi

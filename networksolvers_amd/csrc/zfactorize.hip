@@ -204,6 +204,7 @@ void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int row
     else
       gemm_c128(ctx, zdesc(nc, nc, m, true, false), M, false, M, true, rho.as<double>(), kRoleFact);
   }
+  require_finite(ctx, rho.as<double>(), 2 * iso * iso, "factorize");
   {
     ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
     zcheck(rocsolver_zheevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),

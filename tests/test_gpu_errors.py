@@ -1,0 +1,134 @@
+"""Non-finite input on the device paths, through the C ABI: NaN or Inf in theta,
+in a Krylov start vector or in H_eff comes back as NSMI_ERR_NOT_FINITE (no
+library eigensolver or SVD is handed such values), and the context keeps working
+afterwards.  The reference has no such check (a NaN would surface inside LAPACK);
+the status is this library's own, declared in include/nsmi.h."""
+import numpy as np
+import pytest
+
+from oracle import chain, linalg, models
+
+pytestmark = pytest.mark.gpu
+
+NOT_FINITE = -5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import networksolvers_amd as ns
+    c = ns.Context(0)
+    yield c
+    c.close()
+
+
+def _graded(m, n, seed):
+    rng = np.random.default_rng(seed)
+    r = min(m, n)
+    U0, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    return (U0 * np.exp(-np.arange(r) / 25.0)) @ V0.T
+
+
+def _factorize_still_works(ctx, m, n, route):
+    import networksolvers_amd as ns
+    M = _graded(m, n, 7)
+    U, C, info = ns.factorize(ctx.upload(M.reshape(m, 1, 1, n)), 2, False, cutoff=1e-9, which_decomp=route)
+    Uo, Co, io = linalg.factorize(M, cutoff=1e-9, which_decomp=route)
+    assert info.kept == io.kept
+    np.testing.assert_allclose(U.numpy().reshape(m, -1) @ C.numpy().reshape(-1, n), Uo @ Co, atol=1e-12)
+
+
+# one case per eigensolver path: batched Jacobi (Gram size <= 40), the one-workgroup
+# and the several-workgroup tridiagonal reductions (plain and cooperative launch),
+# dsyevd above 1024, and the direct SVD of small matrices on the "svd" route
+@pytest.mark.parametrize("m,n,route", [(20, 24, "eigen"), (100, 110, "eigen"), (300, 320, "eigen"),
+                                       (700, 700, "eigen"), (1100, 1000, "eigen"), (12, 20, "svd")])
+@pytest.mark.parametrize("bad,where", [(np.nan, "middle"), (np.inf, "middle"), (np.nan, "corner"),
+                                       (-np.inf, "first")])
+def test_factorize_rejects_non_finite_theta(ctx, m, n, route, bad, where):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    M = _graded(m, n, m + n)
+    i, j = {"middle": (m // 3, n // 2), "corner": (m - 1, n - 1), "first": (0, 0)}[where]
+    M[i, j] = bad
+    print(f"factorize non-finite m={m} n={n} route={route} bad={bad} at {where}", flush=True)
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        ns.factorize(ctx.upload(M.reshape(m, 1, 1, n)), 2, False, cutoff=1e-9, which_decomp=route)
+    assert err.value.code == NOT_FINITE
+    _factorize_still_works(ctx, min(m, 64), min(n, 70), route)
+
+
+def test_complex_factorize_rejects_non_finite_theta(ctx):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    rng = np.random.default_rng(3)
+    th = rng.standard_normal((30, 2, 2, 40)) + 1j * rng.standard_normal((30, 2, 2, 40))
+    th[4, 1, 0, 17] = complex(0.0, np.nan)
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        ns.factorize(ctx.upload(th), 2, False, cutoff=1e-10, maxdim=32)
+    assert err.value.code == NOT_FINITE
+    th[4, 1, 0, 17] = 1.0 + 0.5j
+    U, C, info = ns.factorize(ctx.upload(th), 2, False, cutoff=1e-10, maxdim=32)
+    assert np.isfinite(info.truncerr) and info.kept == 32
+
+
+def _tiny_heff(ctx, poison=None, dtype=np.float64):
+    """H_eff of the 2-site XXZ chain (dimension 4), optionally with a NaN or Inf
+    in L; complex environments (the MPO stays real) when dtype is complex128."""
+    import networksolvers_amd as ns
+    W = models.xxz_mpo(2)
+    L = np.full((1, 1, 1), 1.0 if poison is None else poison, dtype)
+    R = np.ones((1, 1, 1), dtype)
+    return ns.HeffMap(ctx.upload(L), ctx.upload(W[0]), ctx.upload(W[1]), ctx.upload(R)), W, L, R
+
+
+def _lanczos_still_works(ctx):
+    import networksolvers_amd as ns
+    H, W, L, R = _tiny_heff(ctx)
+    w = np.linalg.eigvalsh(chain.heff2_dense_matrix(L, W[0], W[1], R))
+    x = ctx.upload(np.random.default_rng(5).standard_normal((1, 2, 2, 1)))
+    res = ns.lanczos_eig(H, x, krylovdim=4, tol=1e-14)
+    assert abs(res.eigval - w[0]) < 1e-12
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_lanczos_rejects_non_finite_start_vector(ctx, bad):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    H, *_ = _tiny_heff(ctx)
+    x0 = np.random.default_rng(1).standard_normal((1, 2, 2, 1))
+    x0[0, 1, 0, 0] = bad
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        ns.lanczos_eig(H, ctx.upload(x0), krylovdim=3)
+    assert err.value.code == NOT_FINITE
+    _lanczos_still_works(ctx)
+
+
+def test_lanczos_rejects_non_finite_operator(ctx):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    H, *_ = _tiny_heff(ctx, poison=np.nan)
+    x0 = np.random.default_rng(2).standard_normal((1, 2, 2, 1))
+    with pytest.raises(NsmiError, match="non-finite recurrence coefficient") as err:
+        ns.lanczos_eig(H, ctx.upload(x0), krylovdim=3)
+    assert err.value.code == NOT_FINITE
+    _lanczos_still_works(ctx)
+
+
+@pytest.mark.parametrize("tau", [-0.1, -0.1j])
+def test_expm_rejects_non_finite_input(ctx, tau):
+    import networksolvers_amd as ns
+    from networksolvers_amd._lib import NsmiError
+    dt = np.complex128 if isinstance(tau, complex) else np.float64
+    x0 = np.random.default_rng(4).standard_normal((1, 2, 2, 1)).astype(dt)
+    bad = x0.copy()
+    bad[0, 0, 1, 0] = np.nan
+    H, *_ = _tiny_heff(ctx, dtype=dt)
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        ns.krylov_expm(H, ctx.upload(bad), tau, krylovdim=4)
+    assert err.value.code == NOT_FINITE
+    Hp, *_ = _tiny_heff(ctx, poison=np.inf, dtype=dt)
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        ns.krylov_expm(Hp, ctx.upload(x0), tau, krylovdim=4)
+    assert err.value.code == NOT_FINITE
+    _lanczos_still_works(ctx)

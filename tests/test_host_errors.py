@@ -1,0 +1,83 @@
+"""Host-side failure paths of the sweep framework, forced on the CPU: the errors
+the reference raises for an unsupported region arity (inserter.jl:25-26,
+extracter.jl via the chain theta), TDVP order (tdvp_region_plans.jl:10), a
+Runge-Kutta solver without order 2 or 4 (runge_kutta.jl:21-22) and an unknown
+subspace expansion (subspace.jl:16-20).  None of these reaches the device."""
+import pytest
+
+
+def _chain(n=6):
+    from networksolvers_amd.region_plans import path_graph
+    return path_graph(n)
+
+
+def test_tdvp_orders():
+    from networksolvers_amd.region_plans import tdvp_regions, tdvp_sub_time_steps
+    assert tdvp_sub_time_steps(1) == [1.0]
+    assert tdvp_sub_time_steps(2) == [0.5, 0.5]
+    assert abs(sum(tdvp_sub_time_steps(4)) - 1.0) < 1e-14
+    for order in (0, 3, 5):
+        with pytest.raises(ValueError, match=f"TDVP order of {order} not supported"):
+            tdvp_sub_time_steps(order)
+        with pytest.raises(ValueError, match="not supported"):
+            tdvp_regions(_chain(), 0.1, updater_kwargs={}, tdvp_order=order, nsites=2)
+
+
+@pytest.mark.parametrize("nsites", [0, 3])
+def test_plans_reject_region_arity(nsites):
+    from networksolvers_amd.region_plans import euler_sweep, post_order_dfs_plan
+    with pytest.raises(ValueError, match="not supported"):
+        euler_sweep(_chain(), nsites=nsites)
+    with pytest.raises(ValueError, match="not supported"):
+        post_order_dfs_plan(_chain(), nsites=nsites)
+
+
+def test_runge_kutta_requires_order():
+    from networksolvers_amd.applyexp import runge_kutta_solver
+    with pytest.raises(ValueError, match="must specify `order`"):
+        runge_kutta_solver(None, 0.1, None, order=3)
+
+
+def test_exponentiate_requires_hermitian():
+    from networksolvers_amd.applyexp import exponentiate_solver
+    with pytest.raises(NotImplementedError, match="Hermitian"):
+        exponentiate_solver(None, 0.1, None, ishermitian=False)
+
+
+class _Dummy:
+    """Stands for a chain state or operator on paths that must fail before use."""
+
+
+def _region_step(region):
+    from networksolvers_amd.sweep import EigsolveProblem, RegionIterator
+    problem = EigsolveProblem(state=_Dummy(), operator=_Dummy())
+    return problem, RegionIterator(problem, [(region, {})])
+
+
+@pytest.mark.parametrize("region", [[0, 1, 2], [1, 2, 3, 4]])
+def test_inserter_rejects_long_regions(region):
+    from networksolvers_amd.sweep import inserter
+    problem, it = _region_step(region)
+    with pytest.raises(ValueError, match=f"Region of length {len(region)} not currently supported"):
+        inserter(problem, _Dummy(), it, sweep=1)
+
+
+@pytest.mark.parametrize("alg", ["krylov", "bogus", "DensityMatrix"])
+def test_extracter_rejects_unknown_subspace_algorithm(alg):
+    from networksolvers_amd.sweep import extracter
+    problem, it = _region_step([0, 1])
+    with pytest.raises(NotImplementedError, match="Subspace expansion"):
+        extracter(problem, it, sweep=1, subspace_algorithm=alg)
+
+
+def test_status_names_match_header():
+    """Every NSMI_ERR_* status of the header has its name on the host."""
+    import os
+    import re
+    from networksolvers_amd import _lib
+    hdr = os.path.join(os.path.dirname(__file__), "..", "include", "nsmi.h")
+    codes = {int(v): k for k, v in re.findall(r"#define (NSMI_ERR_[A-Z_]+) \((-\d+)\)", open(hdr).read())}
+    assert codes[-5] == "NSMI_ERR_NOT_FINITE"
+    assert codes == _lib.ERRORS
+    err = _lib.NsmiError(-5, "factorize: non-finite entries in the matrix to be factorized")
+    assert "NSMI_ERR_NOT_FINITE" in str(err) and err.code == -5
