@@ -12,10 +12,20 @@ its entries above the cut value docut (so a near-degenerate pair at the cut is
 kept or dropped together).  The new link lists its sectors in increasing charge,
 each with its kept vectors in decreasing spectral order.
 
+The densitymatrix subspace expansion (subspace/densitymatrix.jl) is restated
+the same way: its S is block diagonal in the charge of the expanded link, so rho
+is decomposed per sector and truncated across sectors.
+
 Pinned by (tests/test_oracle_qn.py): without truncation the sector split
 reproduces theta exactly and its merged spectrum equals the dense one; the QN
 ground energy of the spin-1 chain N=10 equals the reference's constant
--12.8945601 and the ED fixture; QN and dense DMRG energies agree.
+-12.8945601 and the ED fixture; QN and dense DMRG energies agree.  With the
+expansion: the reference's QN example (examples/dmrg.jl, conserve_qns=true)
+reaches -12.8945601 and the ED fixture; region by region the QN sweep equals the
+dense expansion of the tree restatement on the path graph (kept dimension exact,
+Ritz value 1e-10, on a disordered chain with no degenerate cut); one expansion
+step keeps the two-site state, leaves [A | U] an isometry and the tensors
+charge-block-sparse.
 """
 from __future__ import annotations
 
@@ -170,6 +180,96 @@ def qn_split(theta, ql, qs, qr_link, rows_trailing, **trunc):
     return left, right, mid, truncerr
 
 
+def densitymatrix_expand_qn(mps, links, qs, cW, W, envs, prev_pos, region, local, *, cutoff, maxdim, mindim,
+                            expansion_factor=1.5, max_expand=None, north_pass=1):
+    """subspace/densitymatrix.jl:5-74 on a QN chain (the 2-site region's previous
+    position p next to it): S = (1 - P_A)(E_p . W_p . A_p) is block diagonal in the
+    charge Q of the link between p and the region, so rho = S S^T is decomposed per
+    sector and truncated across sectors (the block-sparse eigen of a QN ITensor,
+    eigen route, maxdim = compute_expansion); U is projected off A once more and
+    checked (|U^T A| <= 1e-10, else nothing changes); the link gains, in every
+    charge, the old basis followed by that charge's new vectors, A <- [A | U], and
+    the next site and the local tensor are multiplied by expander = [A | U]^T A.
+    Dense tensors throughout, charge-forbidden blocks zero.  Returns (local,
+    changed sites)."""
+    from .tree import compute_expansion
+    lo, hi = min(region), max(region)
+    prev = [v for v in prev_pos if v not in (lo, hi)]
+    if len(prev) != 1 or prev[0] not in (lo - 1, hi + 1):
+        return local, []
+    p = prev[0]
+    left = p == lo - 1                      # A = mps[p] [l, s, a]; otherwise [a, s, r]
+    v = lo if left else hi
+    A = mps[p]
+    ia = p + 1 if left else p               # links[ia] is the link between p and v
+    if left:
+        E = envs.get(mps, p - 1, p)                                       # [bl, wl, l]
+        S = np.einsum("xwl,lsa,wvts->xtav", E, A, W[p])                 # [bl, t, a, wa]
+        Ar = A                                                            # rows (l, s)
+        qrow = links[p].charges()[:, None] + qs[None, :]
+        qcol = links[ia].charges()[:, None] + cW[p + 1][None, :]
+    else:
+        E = envs.get(mps, p + 1, p)                                       # [br, vr, r]
+        S = np.einsum("asr,yvr,wvts->tyaw", A, E, W[p])                 # [t, br, a, wa]
+        Ar = A.transpose(1, 2, 0)                                         # rows (s, r)
+        qrow = -qs[:, None] + links[p + 1].charges()[None, :]
+        qcol = links[ia].charges()[:, None] + cW[p][None, :]
+    n1, n2, dim_a = Ar.shape
+    nexp = compute_expansion(dim_a, n1 * n2, expansion_factor, max_expand, maxdim)
+    if nexp <= 0:
+        return local, []
+    R, C = n1 * n2, S.shape[2] * S.shape[3]
+    A2 = Ar.reshape(R, dim_a)
+    S2 = S.reshape(R, C)
+    for _ in range(north_pass):
+        S2 = S2 - A2 @ (A2.T @ S2)
+    qr_, qc_ = qrow.reshape(-1), qcol.reshape(-1)
+    sectors = sorted(set(qr_.tolist()) & set(qc_.tolist()))
+    ridx = [np.nonzero(qr_ == q)[0] for q in sectors]
+    mats = [S2[np.ix_(ri, np.nonzero(qc_ == q)[0])] for q, ri in zip(sectors, ridx)]
+    res, kept, _ = factorize_sectors(mats, False, cutoff=cutoff, maxdim=nexp, mindim=mindim, which_decomp="eigen")
+    cols, ucharge = [], []
+    for q, ri, r_ in zip(sectors, ridx, res):
+        if r_ is None:
+            continue
+        U = np.zeros((R, r_[0].shape[1]))
+        U[ri] = r_[0]
+        cols.append(U)
+        ucharge += [q] * U.shape[1]
+    if not cols:
+        return local, []
+    U = np.concatenate(cols, axis=1)
+    for _ in range(north_pass):
+        U = U - A2 @ (A2.T @ U)
+    if np.linalg.norm(U.T @ A2) > 1e-10:
+        return local, []
+    old = links[ia]
+    qa = old.charges()
+    ucharge = np.array(ucharge)
+    new_sectors = sorted({q: 0 for q, _ in old.sectors} | {int(q): 0 for q in ucharge})
+    dims = {q: int(np.sum(qa == q)) + int(np.sum(ucharge == q)) for q in new_sectors}
+    mid = Link([(q, dims[q]) for q in new_sectors])
+    mo = mid.offsets()
+    K = sum(dims.values())
+    Ax = np.zeros((R, K))
+    for q in new_sectors:
+        oc = np.nonzero(qa == q)[0]
+        uc = np.nonzero(ucharge == q)[0]
+        Ax[:, mo[q]:mo[q] + len(oc)] = A2[:, oc]
+        Ax[:, mo[q] + len(oc):mo[q] + len(oc) + len(uc)] = U[:, uc]
+    expander = Ax.T @ A2                                                  # [ax, a]
+    links[ia] = mid
+    if left:
+        mps[p] = Ax.reshape(n1, n2, K)
+        mps[v] = np.tensordot(expander, mps[v], axes=([1], [0]))
+        local = np.tensordot(expander, local, axes=([1], [0]))
+    else:
+        mps[p] = Ax.reshape(n1, n2, K).transpose(2, 0, 1).copy()
+        mps[v] = np.tensordot(mps[v], expander, axes=([2], [1]))
+        local = np.tensordot(local, expander, axes=([3], [1]))
+    return local, [p, v]
+
+
 @dataclass
 class QNResult:
     energy: float
@@ -179,10 +279,12 @@ class QNResult:
 
 
 def qn_dmrg_chain(W, states, d, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, krylovdim=3, tol=1e-14,
-                  which_decomp=None):
+                  which_decomp=None, subspace_algorithm=None, expansion_factor=1.5, max_expand=None, north_pass=1):
     """2-site QN DMRG from a product state (basis state states[i] on site i) over
-    the Euler-tour plan from the last site; log rows (region, eigenvalue, kept,
-    truncerr, new link sectors)."""
+    the Euler-tour plan from the last site, with the densitymatrix subspace
+    expansion in the extracter when asked (extracter.jl:10-12, the operator's
+    previous position, the inserter's truncation); log rows (region, eigenvalue,
+    kept, truncerr, new link sectors)."""
     N = len(W)
     qs = spin_charges(d)
     links = [Link([(0, 1)])]
@@ -194,17 +296,29 @@ def qn_dmrg_chain(W, states, d, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, k
         a = np.zeros((1, d, 1))
         a[0, s, 0] = 1.0
         mps.append(a)
+    if subspace_algorithm not in (None, "densitymatrix"):
+        raise NotImplementedError("Subspace expansion (subspace_expand!) not defined for requested combination "
+                                  "of subspace_algorithm and problem types")
+    cW = mpo_link_charges(W, qs)
     envs = ChainEnvs(W)
     log, E = [], None
+    pos = list(range(N))
     for sweep in range(1, nsweeps + 1):
         cut = get_or_last(cutoff, sweep)
         mxd = get_or_last(maxdim, sweep) if maxdim is not None else None
         mnd = get_or_last(mindim, sweep)
         for (u, v) in euler_sweep_chain(N, N - 1):
             lo, hi = min(u, v), max(u, v)
+            th = theta2(mps[lo], mps[hi])
+            if subspace_algorithm == "densitymatrix":
+                th, changed = densitymatrix_expand_qn(mps, links, qs, cW, W, envs, pos, (lo, hi), th, cutoff=cut,
+                                                      maxdim=mxd, mindim=mnd, expansion_factor=expansion_factor,
+                                                      max_expand=max_expand, north_pass=north_pass)
+                for c in changed:
+                    envs.site_changed(c)
+            pos = [lo, hi]
             L = envs.get(mps, lo - 1, lo)
             R = envs.get(mps, hi + 1, hi)
-            th = theta2(mps[lo], mps[hi])
             shape = th.shape
             E, x, _ = linalg.lanczos_eigsolve(lambda y: heff2_apply(L, W[lo], W[hi], R, y.reshape(shape)).reshape(-1),
                                               th.reshape(-1), krylovdim=krylovdim, tol=tol)

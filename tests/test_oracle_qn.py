@@ -80,3 +80,125 @@ def test_qn_equals_dense_dmrg():
     d = chain.dmrg_chain(W, chain.product_mps(states, 2), nsweeps=6, maxdim=[8, 16, 32, 64], cutoff=1e-10)
     assert abs(q.energy - d.energy) < 1e-12
     assert [m.shape[2] for m in q.mps[:-1]] == [m.shape[2] for m in d.mps[:-1]]
+
+
+def _neel_s1(N):
+    """The reference example's Neel start: "Up" (m = +1) on even 1-based sites, "Dn" (m = -1) otherwise."""
+    return [0 if (i + 1) % 2 == 0 else 2 for i in range(N)]
+
+
+def _path_dmrg_dense(W, states, d, **kw):
+    from oracle import tree as otree
+    N = len(W)
+    V, E = list(range(N)), [(i, i + 1) for i in range(N - 1)]
+    Wd = {0: W[0][0], N - 1: W[N - 1][:, 0]}
+    Wd.update({i: W[i] for i in range(1, N - 1)})
+    psi = {}
+    for i, s in enumerate(states):
+        k = 1 if i in (0, N - 1) else 2
+        a = np.zeros([1] * k + [d])
+        a[(0,) * k + (s,)] = 1.0
+        psi[i] = a
+    return otree.tree_dmrg(V, E, Wd, psi, nsites=2, **kw)
+
+
+def test_qn_densitymatrix_reference_example():
+    """examples/dmrg.jl:10-43 with conserve_qns=true: S=1 chain N=10 from the Neel
+    state, 2-site DMRG with densitymatrix expansion (factor 1.1, cutoff 1e-12,
+    maxdim [10,40,80,160], 5 sweeps) reaches the reference's -12.8945601 and the ED
+    fixture, and the expansion grows the links beyond plain 2-site DMRG's."""
+    N = 10
+    W = models.heisenberg_mpo(N, d=3)
+    q = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsweeps=5, maxdim=[10, 40, 80, 160], cutoff=1e-12,
+                          subspace_algorithm="densitymatrix", expansion_factor=1.1)
+    assert abs(q.energy - (-12.8945601)) < 1e-7
+    assert abs(q.energy - _ed("heisenberg_s1_chain", "10")) < 1e-9
+    plain = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsweeps=5, maxdim=[10, 40, 80, 160], cutoff=1e-12)
+    grown = [sum(n for _, n in lk.sectors) for lk in q.links]
+    assert max(grown) > max(sum(n for _, n in lk.sectors) for lk in plain.links)
+
+
+def _disordered_xxz_s1(N, seed):
+    """S=1 XXZ chain (Jz = 0.7 Jxy) with bond couplings J_i in [0.7, 1.3] and fields
+    h_i in [-0.2, 0.2]: the uniform MPO with its coupling row scaled per site."""
+    rng = np.random.default_rng(seed)
+    W = models.xxz_mpo(N, d=3, Jxy=1.0, Jz=0.7, hz=1.0)
+    for i in range(N):
+        r = 0 if i == 0 else 4
+        W[i][r, 0] *= rng.uniform(-0.2, 0.2)
+        if i < N - 1:
+            W[i][r, 1:4] *= rng.uniform(0.7, 1.3)
+    return W
+
+
+@pytest.mark.parametrize("factor,maxdim", [(1.1, [10, 20, 40]), (1.5, [8, 16, 24])])
+def test_qn_expansion_equals_dense_expansion(factor, maxdim):
+    """The expansion's S is block diagonal in the charge of the expanded link, so
+    the sector-wise eigen-decomposition with cross-sector truncation keeps the same
+    space as the dense one: region by region the QN sweep equals the dense
+    expansion on the path graph (kept dimension exact, Ritz value 1e-10).  The
+    chain is an S=1 XXZ chain with random bond couplings and fields, so that no cut
+    meets a (near-)degenerate pair: there dense truncation keeps part of the pair
+    and the block-sparse cut keeps or drops it whole (NDTensors' docut).  The
+    cutoff (1e-8) sits well above the eigensolvers' absolute precision."""
+    N = 10
+    W = _disordered_xxz_s1(N, 5)
+    kw = dict(nsweeps=4, maxdim=maxdim, cutoff=1e-8, subspace_algorithm="densitymatrix", expansion_factor=factor)
+    q = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, **kw)
+    d = _path_dmrg_dense(W, _neel_s1(N), 3, **kw)
+    assert len(q.log) == len(d.log)
+    for a, b in zip(q.log, d.log):
+        assert a[2] == b[2]
+        assert abs(a[1] - b[1]) < 1e-10 * abs(b[1])
+
+
+@pytest.mark.parametrize("left", [True, False])
+def test_qn_expansion_step_invariants(left):
+    """One expansion step on a converged-ish state: [A | U] is an isometry, the
+    two-site product A.theta is unchanged (expander = [A|U]^T A keeps the state),
+    each link charge gains at most its own new vectors, and every tensor keeps zero
+    weight outside the charge-allowed blocks."""
+    N = 8
+    W = models.heisenberg_mpo(N, d=3)
+    qs = oqn.spin_charges(3)
+    res = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsweeps=2, maxdim=6, cutoff=1e-12)
+    mps, links = [m.copy() for m in res.mps], list(res.links)
+    cW = oqn.mpo_link_charges(W, qs)
+    envs = chain.ChainEnvs(W)
+    lo, hi = (4, 5) if left else (2, 3)
+    p = lo - 1 if left else hi + 1
+    ia = p + 1 if left else p
+    # put the gauge centre on the region, as the extracter does: p is an isometry toward it
+    for i in range(N - 1):
+        if i < lo:
+            A = mps[i].reshape(-1, mps[i].shape[2])
+            Q, Rm = np.linalg.qr(A)
+            if Q.shape[1] == A.shape[1]:
+                mps[i] = Q.reshape(mps[i].shape)
+                mps[i + 1] = np.tensordot(Rm, mps[i + 1], axes=([1], [0]))
+    for i in range(N - 1, hi, -1):
+        B = mps[i].reshape(mps[i].shape[0], -1)
+        Q, Rm = np.linalg.qr(B.T)
+        if Q.shape[1] == B.shape[0]:
+            mps[i] = Q.T.reshape(mps[i].shape)
+            mps[i - 1] = np.tensordot(mps[i - 1], Rm.T, axes=([2], [0]))
+    th = chain.theta2(mps[lo], mps[hi])
+    A0, old = mps[p].copy(), links[ia]
+    before = (np.tensordot(A0, th, axes=([2], [0])) if left else np.tensordot(th, A0, axes=([3], [0])))
+    th2, changed = oqn.densitymatrix_expand_qn(mps, links, qs, cW, W, envs, [lo - 1, lo] if left else [hi, hi + 1],
+                                               (lo, hi), th, cutoff=1e-12, maxdim=64, mindim=1)
+    assert sorted(changed) == sorted([p, lo if left else hi])
+    A1 = mps[p]
+    Am = A1.reshape(-1, A1.shape[2]) if left else A1.reshape(A1.shape[0], -1).T
+    np.testing.assert_allclose(Am.T @ Am, np.eye(Am.shape[1]), atol=1e-10)
+    after = (np.tensordot(A1, th2, axes=([2], [0])) if left else np.tensordot(th2, A1, axes=([3], [0])))
+    np.testing.assert_allclose(after, before, atol=1e-10)
+    new = dict(links[ia].sectors)
+    assert sum(new.values()) > sum(n for _, n in old.sectors)
+    for q, n in old.sectors:
+        assert new[q] >= n
+    # charge-allowed support of the new site tensor (the dense QR gauge above leaves
+    # round-off in forbidden blocks)
+    ql, qa = (links[p].charges(), links[p + 1].charges())
+    allowed = ql[:, None, None] + qs[None, :, None] == qa[None, None, :]
+    assert np.max(np.abs(A1[~allowed]), initial=0.0) < 1e-14
