@@ -331,15 +331,42 @@ def _boundary_env(ctx, link: QNIndex, mpo_link: QNIndex):
     return E
 
 
-def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
-    """E'[b', w', b] = sum conj(A[a', s', b']) E[a', w, a] W[w, w', s', s] A[a, s, b]
-    (extracter.jl:14 -> ProjTTN environment) as a block program: X = E.A (GEMM),
-    Y = W-mix (LINCOMB), E' = A^T.Y (GEMM with segments over (a', s'))."""
+class _Mix:
+    """The first two steps of an environment update, planned on the host: X = E.A
+    (left) or A.E^T (right) as one grouped GEMM, then the MPO mix Y = sum_c c X as
+    one grouped linear combination.  ``Y`` maps a block key to its offset in the
+    Y scratch buffer and ``ysize`` to its element count."""
+
+    def __init__(self, left, xs, X, X_off, ys, ysize):
+        self.left, self.xs, self.X, self.X_off, self.ys, self.ysize = left, xs, X, X_off, ys, ysize
+        self.Y, off = {}, 0
+        for k in ys:
+            self.Y[k] = off
+            off += ysize[k]
+        self.Y_off = off
+
+    def emit(self, p, xE, xA):
+        """Appends the two steps to program p (every operand already registered);
+        returns the buffer id of Y."""
+        j0 = len(p.scratch)
+        p.scratch_buf(self.X_off)
+        p.scratch_buf(self.Y_off)
+        sX, sY = p.sid(j0), p.sid(j0 + 1)
+        if self.left:
+            p.gemm(False, False, [(sX, self.X[k], nb, na_p, nb, [(xE, eo, na, xA, ao, nb, na, 1.0)])
+                                  for (k, eo, ao, na_p, na, nb) in self.xs])
+        else:
+            p.gemm(False, True, [(sX, self.X[k], nbp, na, nbp, [(xA, ao, nb, xE, eo, nb, nb, 1.0)])
+                                 for (k, ao, eo, na, nb, nbp) in self.xs])
+        p.lincomb([(sY, self.Y[k], self.ysize[k], [(sX, self.X[xk], c) for xk, c in tl])
+                   for k, tl in self.ys.items()])
+        return sY
+
+
+def _mix_left(E: QNTensor, A: QNTensor, W, cl) -> _Mix:
+    """Y[(ap, sp, wp, b)] = sum_{w, s} W[w, wp, sp, s] (E[ap, w, a] A[a, s, b]), blocks
+    l.n[ap] x r.n[b], for A[l, s, r] and the environment E[l', w, l] on its left."""
     l, site, r = A.indices
-    mw_r = QNIndex([(int(q), 1) for q in cr])
-    out = QNTensor(ctx, [r, mw_r, r], [1, -1, -1], 0)
-    p = _Program()
-    xE, xA = p.operand(E.data), 0
     X, X_off, xs = {}, 0, []
     for (a, s, b) in A.keys:
         for w in range(len(cl)):
@@ -349,22 +376,51 @@ def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
             X[(ap, w, s, b)] = X_off
             xs.append(((ap, w, s, b), E.block_off((ap, w, a)), A.block_off((a, s, b)), l.n[ap], l.n[a], r.n[b]))
             X_off += l.n[ap] * r.n[b]
-    Y, Y_off, ys = {}, 0, {}
-    for (ap, w, s, b), _ in X.items():
+    ys = {}
+    for (ap, w, s, b) in X:
         for wp in range(W.shape[1]):
             for sp in range(W.shape[2]):
                 c = W[w, wp, sp, s]
                 if c != 0.0:
                     ys.setdefault((ap, sp, wp, b), []).append(((ap, w, s, b), c))
-    for k in ys:
-        Y[k] = Y_off
-        Y_off += l.n[k[0]] * r.n[k[3]]
-    p.scratch_buf(X_off)
-    p.scratch_buf(Y_off)
-    sX, sY = p.sid(0), p.sid(1)
-    p.gemm(False, False, [(sX, X[k], nb, na_p, nb, [(xE, eo, na, xA, ao, nb, na, 1.0)])
-                          for (k, eo, ao, na_p, na, nb) in xs])
-    p.lincomb([(sY, Y[k], l.n[k[0]] * r.n[k[3]], [(sX, X[xk], c) for xk, c in tl]) for k, tl in ys.items()])
+    return _Mix(True, xs, X, X_off, ys, {k: l.n[k[0]] * r.n[k[3]] for k in ys})
+
+
+def _mix_right(E: QNTensor, A: QNTensor, W, cr) -> _Mix:
+    """Y[(a, w, sp, bp)] = sum_{v, s} W[w, v, sp, s] (A[a, s, b] E[bp, v, b]), blocks
+    l.n[a] x r.n[bp], for A[l, s, r] and the environment E[r', v, r] on its right."""
+    l, site, r = A.indices
+    X, X_off, xs = {}, 0, []
+    for (a, s, b) in A.keys:
+        for v in range(len(cr)):
+            bp = r.find(r.q[b] + cr[v])
+            if bp is None or not E.has((bp, v, b)):
+                continue
+            X[(a, s, v, bp)] = X_off
+            xs.append(((a, s, v, bp), A.block_off((a, s, b)), E.block_off((bp, v, b)), l.n[a], r.n[b], r.n[bp]))
+            X_off += l.n[a] * r.n[bp]
+    ys = {}
+    for (a, s, v, bp) in X:
+        for w in range(W.shape[0]):
+            for sp in range(W.shape[2]):
+                c = W[w, v, sp, s]
+                if c != 0.0:
+                    ys.setdefault((a, w, sp, bp), []).append(((a, s, v, bp), c))
+    return _Mix(False, xs, X, X_off, ys, {k: l.n[k[0]] * r.n[k[3]] for k in ys})
+
+
+def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
+    """E'[b', w', b] = sum conj(A[a', s', b']) E[a', w, a] W[w, w', s', s] A[a, s, b]
+    (extracter.jl:14 -> ProjTTN environment) as a block program: X = E.A (GEMM),
+    Y = W-mix (LINCOMB), E' = A^T.Y (GEMM with segments over (a', s'))."""
+    l, site, r = A.indices
+    mw_r = QNIndex([(int(q), 1) for q in cr])
+    out = QNTensor(ctx, [r, mw_r, r], [1, -1, -1], 0)
+    mix = _mix_left(E, A, W, cl)
+    p = _Program()
+    xE, xA = p.operand(E.data), 0
+    sY = mix.emit(p, xE, xA)
+    Y = mix.Y
     items = []
     for (bp, wp, b) in out.keys:
         sg = []
@@ -384,34 +440,11 @@ def env_right(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
     l, site, r = A.indices
     mw_l = QNIndex([(int(q), 1) for q in cl])
     out = QNTensor(ctx, [l, mw_l, l], [1, -1, -1], 0)
+    mix = _mix_right(E, A, W, cr)
     p = _Program()
     xE, xA = p.operand(E.data), 0
-    X, X_off, xs = {}, 0, []
-    for (a, s, b) in A.keys:
-        for v in range(len(cr)):
-            bp = r.find(r.q[b] + cr[v])
-            if bp is None or not E.has((bp, v, b)):
-                continue
-            X[(a, s, v, bp)] = X_off
-            xs.append(((a, s, v, bp), A.block_off((a, s, b)), E.block_off((bp, v, b)), l.n[a], r.n[b], r.n[bp]))
-            X_off += l.n[a] * r.n[bp]
-    ys = {}
-    for (a, s, v, bp) in X:
-        for w in range(W.shape[0]):
-            for sp in range(W.shape[2]):
-                c = W[w, v, sp, s]
-                if c != 0.0:
-                    ys.setdefault((a, w, sp, bp), []).append(((a, s, v, bp), c))
-    Y, Y_off = {}, 0
-    for k in ys:
-        Y[k] = Y_off
-        Y_off += l.n[k[0]] * r.n[k[3]]
-    p.scratch_buf(X_off)
-    p.scratch_buf(Y_off)
-    sX, sY = p.sid(0), p.sid(1)
-    p.gemm(False, True, [(sX, X[k], nbp, na, nbp, [(xA, ao, nb, xE, eo, nb, nb, 1.0)])
-                         for (k, ao, eo, na, nb, nbp) in xs])
-    p.lincomb([(sY, Y[k], l.n[k[0]] * r.n[k[3]], [(sX, X[xk], c) for xk, c in tl]) for k, tl in ys.items()])
+    sY = mix.emit(p, xE, xA)
+    Y = mix.Y
     items = []
     for (ap, w, a) in out.keys:
         sg = []
@@ -668,6 +701,312 @@ def insert(psi: QNMPS, region, th: QNTensor, trunc, which_decomp=None):
     return info
 
 
+# ------------------------------------------------------------ subspace expansion
+def _charge_rows(ix1: QNIndex, ix2: QNIndex, sign1, sign2):
+    """Row blocks (i, j) of a matrix over two indices grouped by the charge
+    sign1 q1[i] + sign2 q2[j], in (i, j) order: {charge: ([(i, j)], {(i, j): row
+    offset}, rows)}; every block has ix1.n[i] * ix2.n[j] entries."""
+    out = {}
+    for i in range(len(ix1)):
+        for j in range(len(ix2)):
+            q = sign1 * ix1.q[i] + sign2 * ix2.q[j]
+            keys, ro, n = out.setdefault(q, ([], {}, 0))
+            keys.append((i, j))
+            ro[(i, j)] = n
+            out[q] = (keys, ro, n + ix1.n[i] * ix2.n[j])
+    return out
+
+
+def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north_pass=1, expansion_factor=1.5,
+                         max_expand=None, **kws):
+    """subspace/densitymatrix.jl:5-74 on a QN chain: the link between the
+    operator's previous position p and the 2-site region grows by the dominant
+    eigenvectors of rho = S S^T, S = (1 - A A^T)(E_p . W_p . A_p).  S is block
+    diagonal in the charge Q of that link, so one block program forms S sector by
+    sector (the environment update's GEMM and MPO-mix steps, then the projection
+    as grouped GEMMs), ``nsmi_factorize_sectors`` diagonalises the sector Gram
+    matrices with the eigen route and truncates across sectors (maxdim = the
+    expansion size of subspace.jl:28-48), and a second program projects U off A
+    again, forms the check A^T U, A <- [A | U], expander = [A | U]^T A, and applies
+    the expander to the next site and the local tensor.  In every charge the new
+    link lists the old basis first, then that charge's new vectors.  When
+    |A^T U| > 1e-10 the reference prints a warning and changes nothing; so does
+    this."""
+    from .subspace import compute_expansion
+    psi, P = problem.state, problem.operator
+    lo, hi = min(region), max(region)
+    prev = [v for v in P.sites() if v not in (lo, hi)]
+    if len(prev) != 1 or prev[0] not in (lo - 1, hi + 1):
+        return problem, local_state
+    p = prev[0]
+    left = p == lo - 1
+    v = lo if left else hi
+    ctx = psi.ctx
+    A, C = psi[p], psi[v]
+    site = psi.site
+    if left:
+        lx, _, a = A.indices                      # A[l, s, a]; rows (l, s)
+        rows = _charge_rows(lx, site, 1, 1)
+        E = P.environment(psi, (p - 1, p))
+        mix = _mix_left(E, A, P.H.W[p], P.H.c[p])
+        # Y block (ap, sp, wp, b): row (ap, sp), column (b, wp)
+        yrow = lambda k: (k[0], k[1])             # noqa: E731
+        ycol = lambda k: (k[3], k[2])             # noqa: E731
+        ablk = lambda i, j, b: (i, j, b)          # noqa: E731  A's block of row (i, j) in a-sector b
+    else:
+        a, _, rx = A.indices                      # A[a, s, r]; rows (s, r)
+        rows = _charge_rows(site, rx, -1, 1)
+        E = P.environment(psi, (p + 1, p))
+        mix = _mix_right(E, A, P.H.W[p], P.H.c[p + 1])
+        # Y block (a, w, sp, bp): row (sp, bp), column (a, w)
+        yrow = lambda k: (k[2], k[3])             # noqa: E731
+        ycol = lambda k: (k[0], k[1])             # noqa: E731
+        ablk = lambda i, j, b: (b, i, j)          # noqa: E731
+    dim_a = a.dim
+    basis = (lx.dim if left else rx.dim) * site.dim
+    nexp = compute_expansion(dim_a, basis, expansion_factor=expansion_factor, max_expand=max_expand,
+                             maxdim=trunc.get("maxdim"))
+    if nexp <= 0:
+        return problem, local_state
+    # sector layout: rows of every charge, columns (and so a matrix) where Y has blocks
+    cols = {}
+    for k in mix.Y:
+        q = (lx.q[k[0]] + site.q[k[1]]) if left else (rx.q[k[3]] - site.q[k[2]])
+        cols.setdefault(q, {})[ycol(k)] = None
+    colw = (lambda c: a.n[c[0]])                  # noqa: E731  width of column block (b, w)
+    sectors = sorted(q for q in cols if q in rows)
+    co, C_ = {}, {}
+    for q in sectors:
+        off = 0
+        for c in sorted(cols[q]):
+            co[(q, c)] = off
+            off += colw(c)
+        C_[q] = off
+    na = {q: (a.n[a.find(q)] if a.find(q) is not None else 0) for q in set(rows) | set(a.q)}
+    aq = sorted(q for q in a.q if q in rows)       # charges of A's columns (A_Q matrices)
+    M_off, tot = {}, 0
+    for q in sectors:
+        M_off[q] = tot
+        tot += rows[q][2] * C_[q]
+    A_off, totA = {}, 0
+    for q in aq:
+        A_off[q] = totA
+        totA += rows[q][2] * na[q]
+    Mall = ctx.empty((max(tot, 1),))
+    Aall = ctx.empty((max(totA, 1),))
+    # ---- program 1: S by sectors, A_Q, projection S_Q -= A_Q (A_Q^T S_Q)
+    # left:  M_Q = S_Q  (rows x cols),  A_Q stored rows x na
+    # right: M_Q = S_Q^T (cols x rows), A_Q stored transposed, na x rows
+    p1 = _Program()
+    xE, xA, yM, yA = p1.operand(E.data), p1.operand(A.data), p1.operand(Mall), p1.operand(Aall)
+    sY = mix.emit(p1, xE, xA)
+    proj = [q for q in sectors if na[q] > 0]
+    x2off, zoff, t = {}, {}, 0
+    for q in proj:
+        x2off[q] = t
+        t += na[q] * C_[q]
+    j0 = len(p1.scratch)
+    p1.scratch_buf(t)
+    p1.scratch_buf(tot)
+    sX2, sZ = p1.sid(j0), p1.sid(j0 + 1)
+    zero = [(yM, M_off[q], rows[q][2] * C_[q], []) for q in sectors]
+    zero += [(yA, A_off[q], rows[q][2] * na[q], []) for q in aq]
+    if zero:
+        p1.lincomb(zero)
+    cp = []
+    for k, yo in mix.Y.items():
+        rk, ck = yrow(k), ycol(k)
+        q = (lx.q[k[0]] + site.q[k[1]]) if left else (rx.q[k[3]] - site.q[k[2]])
+        R, ro = rows[q][2], rows[q][1][rk]
+        if left:   # Y block l.n[ap] x a.n[b] at (ro, co)
+            cp.append((sY, yo, a.n[k[3]], yM, M_off[q] + ro * C_[q] + co[(q, ck)], C_[q], lx.n[k[0]], a.n[k[3]]))
+        else:      # Y block a.n[a] x r.n[bp] at (co, ro) of the transposed layout
+            cp.append((sY, yo, rx.n[k[3]], yM, M_off[q] + co[(q, ck)] * R + ro, R, a.n[k[0]], rx.n[k[3]]))
+    for q in aq:
+        b = a.find(q)
+        R = rows[q][2]
+        for (i, j) in rows[q][0]:
+            key = ablk(i, j, b)
+            if not A.has(key):
+                continue
+            ro = rows[q][1][(i, j)]
+            if left:   # A block l.n[i] x na (site j) at rows ro
+                cp.append((xA, A.block_off(key), na[q], yA, A_off[q] + ro * na[q], na[q], lx.n[i], na[q]))
+            else:      # A block na x r.n[j] (site i) at columns ro of the na x R layout
+                cp.append((xA, A.block_off(key), rx.n[j], yA, A_off[q] + ro, R, na[q], rx.n[j]))
+    if cp:
+        p1.copy(cp)
+    for _ in range(north_pass if proj else 0):
+        if left:
+            p1.gemm(True, False, [(sX2, x2off[q], C_[q], na[q], C_[q],
+                                   [(yA, A_off[q], na[q], yM, M_off[q], C_[q], rows[q][2], 1.0)]) for q in proj])
+            p1.gemm(False, False, [(sZ, M_off[q], C_[q], rows[q][2], C_[q],
+                                    [(yA, A_off[q], na[q], sX2, x2off[q], C_[q], na[q], 1.0)]) for q in proj])
+        else:
+            R = {q: rows[q][2] for q in proj}
+            p1.gemm(False, True, [(sX2, x2off[q], na[q], C_[q], na[q],
+                                   [(yM, M_off[q], R[q], yA, A_off[q], R[q], R[q], 1.0)]) for q in proj])
+            p1.gemm(False, False, [(sZ, M_off[q], R[q], C_[q], R[q],
+                                    [(sX2, x2off[q], na[q], yA, A_off[q], R[q], na[q], 1.0)]) for q in proj])
+        p1.lincomb([(yM, M_off[q], rows[q][2] * C_[q], [(yM, M_off[q], 1.0), (sZ, M_off[q], -1.0)])
+                    for q in proj])
+    _run_once(p1, ctx, None, None)
+    # ---- the truncated eigen-decomposition of rho by sectors
+    Rl = [rows[q][2] for q in sectors]
+    Cl = [C_[q] for q in sectors]
+    Uall, _, ks, _ = _factorize_sectors(ctx, Mall, [M_off[q] for q in sectors], Rl if left else Cl,
+                                        Cl if left else Rl, not left, trunc.get("cutoff", 0.0), nexp,
+                                        trunc.get("mindim", 1), "eigen")
+    kq = {q: k for q, k in zip(sectors, ks) if k > 0}
+    if not kq:
+        return problem, local_state
+    U_off, t = {}, 0
+    for q, k in zip(sectors, ks):
+        U_off[q] = t
+        t += k * rows[q][2]
+    # ---- program 2: U projection, the check, [A | U], the expander and its products
+    # left: U_Q rows x k;  right: V_Q = U_Q^T, k x rows
+    new_q = sorted(set(a.q) | set(kq))
+    ax = QNIndex([(q, na.get(q, 0) + kq.get(q, 0)) for q in new_q])
+    if left:
+        newA = QNTensor(ctx, [lx, site, ax], [1, 1, -1], 0)
+        newC = QNTensor(ctx, [ax] + C.indices[1:], [1, 1, -1], 0)
+        newT = QNTensor(ctx, [ax] + local_state.indices[1:], [1, 1, 1, -1], 0)
+    else:
+        newA = QNTensor(ctx, [ax, site, rx], [1, 1, -1], 0)
+        newC = QNTensor(ctx, C.indices[:2] + [ax], [1, 1, -1], 0)
+        newT = QNTensor(ctx, local_state.indices[:3] + [ax], [1, 1, 1, -1], 0)
+    both = [q for q in kq if na[q] > 0]            # charges where U meets A
+    c_off, t = {}, 0
+    for q in both:
+        c_off[q] = t
+        t += na[q] * kq[q]
+    chk = ctx.empty((max(t, 1),))
+    ex_q = [q for q in new_q if na.get(q, 0) > 0]   # expander blocks (n_ax x na)
+    ex_off, te = {}, 0
+    for q in ex_q:
+        ex_off[q] = te
+        te += (na[q] + kq.get(q, 0)) * na[q]
+    p2 = _Program()
+    xAa, xU, xA2, xC, xT = p2.operand(Aall), p2.operand(Uall), p2.operand(A.data), p2.operand(C.data), \
+        p2.operand(local_state.data)
+    yNA, yNC, yNT, yK = p2.operand(newA.data), p2.operand(newC.data), p2.operand(newT.data), p2.operand(chk)
+    j0 = len(p2.scratch)
+    p2.scratch_buf(t)
+    p2.scratch_buf(sum(kq[q] * rows[q][2] for q in both))
+    p2.scratch_buf(te)
+    sX3, sZ3, sEx = p2.sid(j0), p2.sid(j0 + 1), p2.sid(j0 + 2)
+    z_off, tz = {}, 0
+    for q in both:
+        z_off[q] = tz
+        tz += kq[q] * rows[q][2]
+
+    def at_u(out_buf, out_off):   # A_Q^T U_Q (left) or V_Q A_Q (right): na x k / k x na
+        if left:
+            return [(out_buf, out_off[q], kq[q], na[q], kq[q],
+                     [(xAa, A_off[q], na[q], xU, U_off[q], kq[q], rows[q][2], 1.0)]) for q in both]
+        return [(out_buf, out_off[q], na[q], kq[q], na[q],
+                 [(xU, U_off[q], rows[q][2], xAa, A_off[q], rows[q][2], rows[q][2], 1.0)]) for q in both]
+
+    for _ in range(north_pass if both else 0):
+        if left:
+            p2.gemm(True, False, at_u(sX3, c_off))
+            p2.gemm(False, False, [(sZ3, z_off[q], kq[q], rows[q][2], kq[q],
+                                    [(xAa, A_off[q], na[q], sX3, c_off[q], kq[q], na[q], 1.0)]) for q in both])
+        else:
+            p2.gemm(False, True, at_u(sX3, c_off))
+            p2.gemm(False, False, [(sZ3, z_off[q], rows[q][2], kq[q], rows[q][2],
+                                    [(sX3, c_off[q], na[q], xAa, A_off[q], rows[q][2], na[q], 1.0)]) for q in both])
+        p2.lincomb([(xU, U_off[q], kq[q] * rows[q][2], [(xU, U_off[q], 1.0), (sZ3, z_off[q], -1.0)])
+                    for q in both])
+    if both:
+        p2.gemm(left, not left, at_u(yK, c_off))
+    # [A | U]: zero, then A's blocks and U's rows / columns
+    p2.lincomb([(yNA, 0, newA.size, [])])
+    cp = []
+    for key in newA.keys:
+        m = key[2] if left else key[0]
+        q = ax.q[m]
+        i, j = (key[0], key[1]) if left else (key[1], key[2])
+        n_old, k = na.get(q, 0), kq.get(q, 0)
+        nax = ax.n[m]
+        b = a.find(q)
+        if n_old and A.has(ablk(i, j, b)):
+            src = A.block_off(ablk(i, j, b))
+            if left:
+                cp.append((xA2, src, n_old, yNA, newA.block_off(key), nax, lx.n[i], n_old))
+            else:
+                cp.append((xA2, src, rx.n[j], yNA, newA.block_off(key), rx.n[j], n_old, rx.n[j]))
+        if k:
+            ro = rows[q][1][(i, j)]
+            if left:
+                cp.append((xU, U_off[q] + ro * k, k, yNA, newA.block_off(key) + n_old, nax, lx.n[i], k))
+            else:
+                R = rows[q][2]
+                cp.append((xU, U_off[q] + ro, R, yNA, newA.block_off(key) + n_old * rx.n[j], rx.n[j], k, rx.n[j]))
+    if cp:
+        p2.copy(cp)
+    # expander E_Q = [A_Q | U_Q]^T A_Q: na x na on top, then k x na
+    items = []
+    for q in ex_q:
+        R, n_old, k = rows[q][2], na[q], kq.get(q, 0)
+        if left:
+            items.append((sEx, ex_off[q], n_old, n_old, n_old, [(xAa, A_off[q], n_old, xAa, A_off[q], n_old, R, 1.0)]))
+            if k:
+                items.append((sEx, ex_off[q] + n_old * n_old, n_old, k, n_old,
+                              [(xU, U_off[q], k, xAa, A_off[q], n_old, R, 1.0)]))
+    if items:
+        p2.gemm(True, False, items)
+    items = []
+    for q in ex_q:
+        R, n_old, k = rows[q][2], na[q], kq.get(q, 0)
+        if not left:
+            items.append((sEx, ex_off[q], n_old, n_old, n_old, [(xAa, A_off[q], R, xAa, A_off[q], R, R, 1.0)]))
+            if k:
+                items.append((sEx, ex_off[q] + n_old * n_old, n_old, k, n_old,
+                              [(xU, U_off[q], R, xAa, A_off[q], R, R, 1.0)]))
+    if items:
+        p2.gemm(False, True, items)
+    # next site and local tensor: E . C on the left link, or C . E^T on the right link
+    for T_old, T_new, y_new, x_old in ((C, newC, yNC, xC), (local_state, newT, yNT, xT)):
+        items = []
+        for key in T_new.keys:
+            m = key[0] if left else key[-1]
+            q = ax.q[m]
+            nax, n_old = ax.n[m], na.get(q, 0)
+            b = a.find(q)
+            okey = ((b,) + key[1:]) if left else (key[:-1] + (b,))
+            sg = []
+            shp = T_new.shapes[T_new.pos[key]]
+            if left:
+                w = int(np.prod(shp[1:]))
+                if n_old and T_old.has(okey):
+                    sg.append((sEx, ex_off[q], n_old, x_old, T_old.block_off(okey), w, n_old, 1.0))
+                items.append((y_new, T_new.block_off(key), w, nax, w, sg))
+            else:
+                h = int(np.prod(shp[:-1]))
+                if n_old and T_old.has(okey):
+                    sg.append((x_old, T_old.block_off(okey), n_old, sEx, ex_off[q], n_old, n_old, 1.0))
+                items.append((y_new, T_new.block_off(key), nax, h, nax, sg))
+        if items:
+            if left:
+                p2.gemm(False, False, items)
+            else:
+                p2.gemm(False, True, items)
+    _run_once(p2, ctx, None, None)
+    if both:
+        err = chk.norm()
+        if err > 1e-10:
+            print(f"Warning: |U*A| = {err:.3E} in subspace expansion")
+            return problem, local_state
+    ia = p + 1 if left else p
+    psi.links[ia] = ax
+    psi[p] = newA
+    psi[v] = newC
+    problem.state = psi
+    return problem, newT
+
+
 def expect_mpo(psi: QNMPS, H: QNMPO) -> float:
     """<psi|H|psi> / <psi|psi> by a left-to-right environment pass (and one with the
     identity operator for the norm)."""
@@ -685,4 +1024,4 @@ def expect_mpo(psi: QNMPS, H: QNMPO) -> float:
 
 
 __all__ = ["QNIndex", "QNTensor", "QNMPO", "QNMPS", "QNProjTTN", "spin_site_index", "qn_product_mps", "theta",
-           "env_left", "env_right", "heff2", "factorize", "expect_mpo", "BlockProgram"]
+           "env_left", "env_right", "heff2", "factorize", "expect_mpo", "BlockProgram", "densitymatrix_expand"]

@@ -8,7 +8,8 @@ off that vertex's current span; any other name raises the reference's error.
 
 All contractions are device tensordots over labelled indices, so chains (site
 tensor A[l, s, r], boundary links of dimension 1) and trees (A[links..., s]) share
-one implementation through a small label adapter.
+one implementation through a small label adapter; QN chains take the block-sparse
+form in ``qn.densitymatrix_expand``.
 """
 from __future__ import annotations
 
@@ -16,7 +17,7 @@ import math
 
 import numpy as np
 
-from . import kernels
+from . import kernels, qn
 from .tree import TreeTTN, _bra, _permute_to, _tdot
 
 DEFAULT_EXPANSION_FACTOR = 1.5          # subspace.jl:5
@@ -109,6 +110,9 @@ def densitymatrix_expand(problem, local_state, region, *, trunc, north_pass=1,
                          expansion_factor=DEFAULT_EXPANSION_FACTOR, max_expand=DEFAULT_MAX_EXPAND, **kws):
     """densitymatrix.jl:53-122 on the device.  Returns (problem, local_state)."""
     psi, P = problem.state, problem.operator
+    if isinstance(psi, qn.QNMPS):
+        return qn.densitymatrix_expand(problem, local_state, region, trunc=trunc, north_pass=north_pass,
+                                       expansion_factor=expansion_factor, max_expand=max_expand)
     view = _TreeView(psi, P) if isinstance(psi, TreeTTN) else _ChainView(psi, P)
     region = list(region)
     prev = [v for v in P.sites() if v not in region]

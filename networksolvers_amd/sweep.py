@@ -194,12 +194,10 @@ def extracter(problem, region_iter, *, sweep, trunc=None, subspace_algorithm=Non
             "Subspace expansion (subspace_expand!) not defined for requested combination of "
             f"subspace_algorithm={subspace_algorithm!r} and problem types")
     if isinstance(state(problem), qn.QNMPS):
-        if subspace_algorithm is not None:
-            raise NotImplementedError("Subspace expansion (subspace_expand!) not defined for requested combination "
-                                      "of subspace_algorithm and problem types (QN chains)")
-        psi = state(problem)
-        local_state = qn.extract(psi, list(region))
-        problem.operator = operator(problem).position(psi, list(region))
+        local_state = qn.extract(state(problem), list(region))
+        problem, local_state = subspace.subspace_expand(problem, local_state, region,
+                                                        subspace_algorithm=subspace_algorithm, trunc=t, **kws)
+        problem.operator = operator(problem).position(problem.state, list(region))
         return problem, local_state
     if isinstance(state(problem), tree.TreeTTN):
         psi = state(problem)
