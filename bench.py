@@ -247,9 +247,11 @@ def _roofline(stats, cls, total_ms, peak=PEAK_TFLOPS["fp64"]):
 
 def qn_workload(args):
     """Block-sparse U(1) DMRG: spin-1 Heisenberg chain N=64, MPO D=5, total Sz=0
-    from a Neel product state; untimed 2-site sweeps on the maxdim ramp 32, 64,
-    128, 256, 512, then maxdim, then the timed sweeps at maxdim (H_eff block
-    programs in --precision, environments and factorisation FP64, Lanczos K=3);
+    from a Neel product state grown by the densitymatrix subspace expansion
+    (expansion factor 1.5, in every sweep; it stops adding vectors once a link
+    reaches maxdim); untimed 2-site sweeps on the maxdim ramp 32, 64, 128, 256,
+    512, then maxdim, then the timed sweeps at maxdim (H_eff block programs in
+    --precision, environments, expansion and factorisation FP64, Lanczos K=3);
     one further FP64-plain sweep from the same state gives the FP64 throughput and
     |dE/E| between the forms (FP64 <psi|H|psi>)."""
     import networksolvers_amd as ns
@@ -275,9 +277,10 @@ def qn_workload(args):
             print(f"bench:   qn sweep {sweep}/{nsweeps} {now - t[0]:.2f} s  E={p.eigenvalue:.12f}  "
                   f"maxlinkdim={p.state.maxlinkdim()}", file=sys.stderr, flush=True)
             t[0] = now
+        trunc = dict(cutoff=args.cutoff, maxdim=maxdim)
         return eigsolve_problem(prob, nsweeps=n, nsites=2, sweep_printer=progress,
-                                updater_kwargs=dict(precision=precision),
-                                inserter_kwargs=dict(trunc=dict(cutoff=args.cutoff, maxdim=maxdim)))
+                                extracter_kwargs=dict(trunc=trunc, subspace_algorithm="densitymatrix"),
+                                updater_kwargs=dict(precision=precision), inserter_kwargs=dict(trunc=trunc))
 
     if sched:
         sweeps(len(sched), sched)
@@ -311,7 +314,8 @@ def qn_workload(args):
     total_ms = 1e3 * t_timed
     roof, share = _roofline(stats, "block", 1e3 * t_prof, PEAK_TFLOPS[args.precision])
     config = {"workload": f"2-site U(1) block-sparse DMRG, spin-1 Heisenberg chain N={N}, d=3, MPO D=5, total Sz=0 "
-                          f"(Neel start), maxdim {args.maxdim}, cutoff {args.cutoff:g}, Lanczos K=3",
+                          f"(Neel start, densitymatrix subspace expansion), maxdim {args.maxdim}, "
+                          f"cutoff {args.cutoff:g}, Lanczos K=3",
               "precision": args.precision, "regions_per_sweep": 2 * (N - 1), "maxlinkdim_timed": psi.maxlinkdim(),
               "max_sectors_per_link": nsec, "energy": E_psi, "parallelism": "single GPU"}
     line = _common(args, "QN DMRG sweeps/s (spin-1 N=64, Sz=0)", args.steps / t_timed, "sweeps/s", total_ms,

@@ -1,11 +1,14 @@
 // Truncated factorisation theta = U C (ITensors factorize, inserter.jl:23) and the
 // QR gauge walk (extracter.jl:6).  FP64 throughout.
 //
-// "eigen" route: rho = M M^T by MFMA GEMM, dense symmetric eigensolver
-// (rocSOLVER dsyevd), spectrum sorted by decreasing |lambda| on the host,
-// NDTensors truncate! restated on the host, then the kept eigenvectors are
-// gathered with the gauge fix and C = U^T M by MFMA GEMM.
-// "svd" route: rocSOLVER dgesvd of M, P = s^2, same truncation and gauge.
+// Both routes diagonalise rho = M M^T (isometry on the rows; M^T M on the
+// columns) formed by MFMA GEMM: batched Jacobi for small rho, the one-launch
+// tridiagonal reduction with rocSOLVER dstedc / dormtr up to 1024, dsyevd above.
+// "eigen" sorts the spectrum by decreasing |lambda|; "svd" takes P = s^2 =
+// max(lambda, 0) in decreasing order (its P is resolved to eps * P_max
+// absolutely, not relatively as by an SVD of M).  NDTensors truncate! is
+// restated on the host; the kept eigenvectors are gathered with the gauge fix
+// and C = U^T M by MFMA GEMM.
 // rocSOLVER is column-major: a row-major m x n buffer is its n x m transpose.
 #include <rocsolver/rocsolver.h>
 
@@ -180,9 +183,6 @@ __global__ void k_diag_sign(const double* __restrict__ X, int64_t ld, int64_t r,
 
 inline unsigned blocks(int64_t n, int t = 256) { return unsigned((n + t - 1) / t); }
 
-// largest min(rows, cols) for which the "svd" route calls rocSOLVER dgesvd
-constexpr int64_t kSvdDirectMax = 32;
-
 GemmDesc gdesc(int64_t M, int64_t N, int64_t K, bool ta, bool tb) {
   GemmDesc g{};
   g.M = M;
@@ -224,12 +224,11 @@ struct FactCore {
   bool rows_trailing;
   int route;
   int64_t iso = 0, oth = 0, full = 0;
-  bool gram = false;
   bool tri = false;     // rho holds sytrd reflectors, z the tridiagonal eigenvectors
   bool jacobi = false;  // rho awaits the batched Jacobi solve (eigenvalues into wdev)
   std::vector<double> P;
   std::vector<int> order;
-  std::unique_ptr<Scratch> rho, sv, ub, vb, z, tau, wdev;
+  std::unique_ptr<Scratch> rho, z, tau, wdev;
   FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
                                                                          route(r) {
     iso = rt ? nc : m;
@@ -258,12 +257,11 @@ void fact_spectrum(FactCore& f, std::vector<double>& wh) {
 }
 
 void fact_gram(nsmi_ctx* ctx, FactCore& f) {
-  // rocSOLVER's dgesvd is ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs
-  // 47 ms), so above a small size the "svd" route also diagonalises the Gram
-  // matrix; its spectrum P = s^2 is then resolved to eps * P_max absolutely.
-  f.gram = f.route == NSMI_ROUTE_EIGEN || f.full > kSvdDirectMax;
+  // The "svd" route diagonalises the Gram matrix as well: rocSOLVER's dgesvd is
+  // ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs 47 ms) and 8x slower
+  // than the batched Jacobi solver at n = 32 (4.2 vs 0.55 ms per factorisation).
   const int64_t iso = f.iso, oth = f.oth, m = f.m, nc = f.nc;
-  if (f.gram) {
+  {
     f.rho.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
     {
       ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
@@ -310,35 +308,6 @@ void fact_gram(nsmi_ctx* ctx, FactCore& f) {
     NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
     NSMI_HIP(hipStreamSynchronize(ctx->stream));
     fact_spectrum(f, wh);
-  } else {
-    // SVD of the column-major view M^T (nc x m): M^T = Ut S Vt  =>  M = Vt^T S Ut^T
-    // row-major readings: buffer "vb" (full x m col-major) = left vectors of M as m x full rows;
-    // buffer "ub" (nc x full col-major) = right vectors of M as full x nc rows.
-    const int64_t full = f.full;
-    Scratch a(ctx, sizeof(double) * size_t(m) * nc);
-    f.sv.reset(new Scratch(ctx, sizeof(double) * size_t(full)));
-    f.ub.reset(new Scratch(ctx, sizeof(double) * size_t(nc) * full));
-    f.vb.reset(new Scratch(ctx, sizeof(double) * size_t(full) * m));
-    Scratch e(ctx, sizeof(double) * size_t(std::max<int64_t>(full, 2)));
-    require_finite(ctx, f.M, m * nc, "factorize");
-    NSMI_HIP(hipMemcpyAsync(a.p, f.M, sizeof(double) * m * nc, hipMemcpyDeviceToDevice, ctx->stream));
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 0.0, 0.0);
-      check_rocsolver(rocsolver_dgesvd(ctx->blas, rocblas_svect_singular, rocblas_svect_singular, rocblas_int(nc),
-                                       rocblas_int(m), a.as<double>(), rocblas_int(nc), f.sv->as<double>(),
-                                       f.ub->as<double>(), rocblas_int(nc), f.vb->as<double>(), rocblas_int(full),
-                                       e.as<double>(), rocblas_outofplace, ctx->d_info),
-                      "rocsolver_dgesvd");
-    }
-    check_info(ctx, "rocsolver_dgesvd");
-    std::vector<double> sh(full);
-    NSMI_HIP(hipMemcpyAsync(sh.data(), f.sv->p, sizeof(double) * full, hipMemcpyDeviceToHost, ctx->stream));
-    NSMI_HIP(hipStreamSynchronize(ctx->stream));
-    f.P.resize(full);
-    for (int64_t i = 0; i < full; ++i) {
-      if (!std::isfinite(sh[i])) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite singular value");
-      f.P[i] = sh[i] * sh[i];
-    }
   }
 }
 
@@ -378,14 +347,14 @@ void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
 // "largest-magnitude entry positive" gauge.
 void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
   if (k == 0) return;
-  const int64_t m = f.m, nc = f.nc, iso = f.iso, full = f.full;
+  const int64_t m = f.m, nc = f.nc, iso = f.iso;
   Scratch d_sgn(ctx, sizeof(double) * size_t(std::max<int64_t>(k, 1)));
-  Scratch d_idx(ctx, sizeof(int) * size_t(f.gram ? k : 1));
+  Scratch d_idx(ctx, sizeof(int) * size_t(k));
   // eigenvectors: column-major iso x iso = row-major rows are eigenvectors
-  const double* vecs = f.gram ? f.rho->as<double>() : nullptr;
+  const double* vecs = f.rho->as<double>();
   const int* vidx = d_idx.as<int>();
   std::unique_ptr<Scratch> zk;
-  if (f.gram) {
+  {
     NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
     if (f.tri) {
       // kept eigenvectors of T (columns order[0..k) of z), then Q . Zk with the reflectors
@@ -404,7 +373,7 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
     }
   }
   ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(k) * m * nc, 8.0 * (double(m) * nc + double(k) * (m + nc)));
-  if (f.gram) {
+  {
     hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, vecs, iso, iso, 1, vidx,
                        d_sgn.as<double>());
     NSMI_LAUNCH_CHECK();
@@ -425,30 +394,6 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
     }
     // the host order vector must outlive the copy above
     NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  } else {
-    if (!f.rows_trailing) {
-      // isometry = left vectors (columns of the m x full row-major vb)
-      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.vb->as<double>(), m, full, 0,
-                         (const int*)nullptr, d_sgn.as<double>());
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, f.vb->as<double>(), full,
-                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), (const double*)nullptr, U);
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc,
-                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), f.sv->as<double>(), C);
-      NSMI_LAUNCH_CHECK();
-    } else {
-      // isometry = right vectors (rows of the full x nc row-major ub)
-      hipLaunchKernelGGL(k_gauge_sign, dim3(unsigned(k)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc, nc, 1,
-                         (const int*)nullptr, d_sgn.as<double>());
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * nc)), dim3(256), 0, ctx->stream, f.ub->as<double>(), nc,
-                         (const int*)nullptr, int(k), nc, d_sgn.as<double>(), (const double*)nullptr, U);
-      NSMI_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gather_cols, dim3(blocks(k * m)), dim3(256), 0, ctx->stream, f.vb->as<double>(), full,
-                         (const int*)nullptr, int(k), m, d_sgn.as<double>(), f.sv->as<double>(), C);
-      NSMI_LAUNCH_CHECK();
-    }
   }
 }
 

@@ -744,6 +744,11 @@ def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north
     ctx = psi.ctx
     A, C = psi[p], psi[v]
     site = psi.site
+    a = A.indices[2] if left else A.indices[0]
+    nexp = compute_expansion(a.dim, A.indices[0 if left else 2].dim * site.dim, expansion_factor=expansion_factor,
+                             max_expand=max_expand, maxdim=trunc.get("maxdim"))
+    if nexp <= 0:
+        return problem, local_state
     if left:
         lx, _, a = A.indices                      # A[l, s, a]; rows (l, s)
         rows = _charge_rows(lx, site, 1, 1)
@@ -762,12 +767,6 @@ def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north
         yrow = lambda k: (k[2], k[3])             # noqa: E731
         ycol = lambda k: (k[0], k[1])             # noqa: E731
         ablk = lambda i, j, b: (b, i, j)          # noqa: E731
-    dim_a = a.dim
-    basis = (lx.dim if left else rx.dim) * site.dim
-    nexp = compute_expansion(dim_a, basis, expansion_factor=expansion_factor, max_expand=max_expand,
-                             maxdim=trunc.get("maxdim"))
-    if nexp <= 0:
-        return problem, local_state
     # sector layout: rows of every charge, columns (and so a matrix) where Y has blocks
     cols = {}
     for k in mix.Y:

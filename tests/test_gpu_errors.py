@@ -40,7 +40,7 @@ def _factorize_still_works(ctx, m, n, route):
 
 # one case per eigensolver path: batched Jacobi (Gram size <= 40), the one-workgroup
 # and the several-workgroup tridiagonal reductions (plain and cooperative launch),
-# dsyevd above 1024, and the direct SVD of small matrices on the "svd" route
+# dsyevd above 1024, and a small matrix on the "svd" route
 @pytest.mark.parametrize("m,n,route", [(20, 24, "eigen"), (100, 110, "eigen"), (300, 320, "eigen"),
                                        (700, 700, "eigen"), (1100, 1000, "eigen"), (12, 20, "svd")])
 @pytest.mark.parametrize("bad,where", [(np.nan, "middle"), (np.inf, "middle"), (np.nan, "corner"),

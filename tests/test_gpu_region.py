@@ -220,10 +220,8 @@ def test_factorize(ctx, route, trailing, shape, cutoff, maxdim):
     # vectors agree element-wise (same gauge) where the spectrum resolves them:
     # eigen-route eigenvectors of small eigenvalues are only determined to eps/gap
     P = io.spectrum[:k]
-    # the device "svd" route calls dgesvd only up to min(rows, cols) = 32 and
-    # diagonalises the Gram matrix above that (same resolution as "eigen")
-    direct_svd = route == "svd" and min(m, n) <= 32
-    good = P >= (1e-12 if direct_svd else 1e-6) * P[0]
+    # the device "svd" route diagonalises the Gram matrix (same resolution as "eigen")
+    good = P >= 1e-6 * P[0]
     np.testing.assert_allclose(Ug[:, good], Uo[:, good], atol=1e-9)
     np.testing.assert_allclose(Cg[good], Co[good], atol=1e-9)
 
