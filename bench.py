@@ -30,9 +30,10 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", default="chain", choices=["chain", "qn", "tree-tdvp"],
+    p.add_argument("--workload", default="chain", choices=["chain", "qn", "tree-tdvp", "readme"],
                    help="chain: dense XXZ N=100 DMRG (the headline); qn: U(1) spin-1 N=64 block-sparse DMRG; "
-                        "tree-tdvp: 2-site TDVP quench of the TFIM on a 63-site binary tree")
+                        "tree-tdvp: 2-site TDVP quench of the TFIM on a 63-site binary tree; readme: the "
+                        "reference's README timing workload (spin-1 Heisenberg N=100, 5 sweeps, four forms)")
     p.add_argument("--steps", type=int, default=None,
                    help="timed sweeps (chain, qn: default 3) or time steps (tree-tdvp: default 20)")
     p.add_argument("--warmup", type=int, default=None,
@@ -325,6 +326,76 @@ def qn_workload(args):
     return line
 
 
+# The reference's own wall times for this workload (README.md:169-181, the
+# optimal_map code state; hardware unstated; Julia @time of the whole dmrg call).
+README_SECONDS = {"2site_dense": 5.8, "2site_qn": 5.5, "1site_dense_expand": 9.8, "1site_qn_expand": 7.7}
+
+
+def readme_workload(args):
+    """The reference's README timing workload (examples/timed_dmrg/timed_dmrg.jl,
+    README.md:119-181): spin-1 Heisenberg chain N=100 from the Neel state ("Up" on
+    even sites), 5 sweeps, cutoff 1e-9, maxdim [10, 40, 80, 160], in four forms:
+    2-site dense, 2-site U(1) QN, and 1-site with the densitymatrix expansion
+    (max_expand 4) dense and QN.  Each form runs once untimed for one sweep (library
+    start-up), then the whole 5-sweep solve from a fresh product state is timed,
+    environments included, as the reference's @time is.  FP64 plain (the
+    reference's arithmetic); the line's value is the 2-site dense wall time."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import models, qn
+    from networksolvers_amd.sweep import eigsolve_problem
+    ctx = ns.Context(0)
+    N, d = 100, 3
+    W = models.heisenberg_mpo(N, d=d)
+    states = [0 if (i + 1) % 2 == 0 else 2 for i in range(N)]      # "Up" (m=+1) on even 1-based sites
+    maxdim, cutoff, nsw = [10, 40, 80, 160], 1e-9, 5
+    trunc = dict(cutoff=cutoff, maxdim=maxdim)
+    site = qn.spin_site_index(d)
+    H_dense = ns.TTNO(ctx, W)
+    H_qn = qn.QNMPO(ctx, W, site)
+
+    def problem(form):
+        if "qn" in form:
+            return ns.EigsolveProblem(state=qn.qn_product_mps(ctx, states, site),
+                                      operator=ns.sweep.projected_operator(H_qn))
+        return ns.EigsolveProblem(state=ns.TTN.from_arrays(ctx, models.product_mps(states, d)),
+                                  operator=ns.ProjTTN(H_dense))
+
+    def solve(form, nsweeps):
+        nsites = 1 if form.startswith("1site") else 2
+        ek = dict(trunc=trunc)
+        if "expand" in form:
+            ek.update(subspace_algorithm="densitymatrix", max_expand=4)
+        prob = problem(form)
+        return eigsolve_problem(prob, nsweeps=nsweeps, nsites=nsites, extracter_kwargs=ek,
+                                inserter_kwargs=dict(trunc=trunc))
+
+    forms = ["2site_dense", "2site_qn", "1site_dense_expand"]
+    if getattr(qn, "SERVES_1SITE", False):
+        forms.append("1site_qn_expand")
+    res = {}
+    for form in forms:
+        solve(form, 1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        E, psi = solve(form, nsw)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        res[form] = {"seconds": round(dt, 4), "energy": E, "maxlinkdim": psi.maxlinkdim(),
+                     "reference_seconds": README_SECONDS[form],
+                     "speedup_vs_reference": round(README_SECONDS[form] / dt, 3)}
+        print(f"bench: readme {form}: {dt:.3f} s for {nsw} sweeps, E={E:.12f}, maxlinkdim={psi.maxlinkdim()} "
+              f"(reference {README_SECONDS[form]} s)", file=sys.stderr, flush=True)
+    ctx.close()
+    v = res["2site_dense"]["seconds"]
+    line = _common(args, "DMRG wall time, spin-1 Heisenberg N=100, 5 sweeps (README timing workload)", v, "s",
+                   1e3 * v, nsw, "f64", {"workload": "S=1 Heisenberg chain N=100, Neel start, cutoff 1e-9, maxdim "
+                                                     "[10,40,80,160], 5 sweeps; 2-site dense timed as the value",
+                                         "parallelism": "single GPU"}, None, None)
+    line.update(higher_is_better=False, vs_baseline=round(v / README_SECONDS["2site_dense"], 4), forms=res,
+                warmup=1)
+    return line
+
+
 def tree_tdvp_workload(args):
     """2-site TDVP quench, order 4, exponentiate solver: H = -sum ZZ - h sum X
     (h = 1) on the complete binary tree of 63 sites, all-up start, dt = 0.05;
@@ -475,7 +546,7 @@ def main():
                                    "would shard by MPO channel or real space; see the design notes)"}))
         sys.exit(2)
     if args.workload != "chain":
-        line = qn_workload(args) if args.workload == "qn" else tree_tdvp_workload(args)
+        line = {"qn": qn_workload, "tree-tdvp": tree_tdvp_workload, "readme": readme_workload}[args.workload](args)
         print(json.dumps(line), flush=True)
         return
     r = gpu_sweeps(args)

@@ -202,13 +202,6 @@ void check_rocsolver(rocblas_status st, const char* what) {
     fail(NSMI_ERR_SOLVER, std::string(what) + " failed with rocblas_status " + std::to_string(int(st)));
 }
 
-void check_info(nsmi_ctx* ctx, const char* what) {
-  int info = 0;
-  NSMI_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  if (info != 0) fail(NSMI_ERR_SOLVER, std::string(what) + " returned info = " + std::to_string(info));
-}
-
 }  // namespace
 }  // namespace nsmi
 
@@ -291,7 +284,6 @@ void fact_gram(nsmi_ctx* ctx, FactCore& f) {
                                          e.as<double>(), f.z->as<double>(), rocblas_int(iso), ctx->d_info),
                         "rocsolver_dstedc");
       }
-      check_info(ctx, "rocsolver_dstedc");
     } else {
       f.tau.reset();
       require_finite(ctx, f.rho->as<double>(), iso * iso, "factorize");
@@ -302,11 +294,16 @@ void fact_gram(nsmi_ctx* ctx, FactCore& f) {
                                          ctx->d_info),
                         "rocsolver_dsyevd");
       }
-      check_info(ctx, "rocsolver_dsyevd");
     }
+    // the solver's status and the eigenvalues in one round trip
     std::vector<double> wh(iso);
+    int info = 0;
+    NSMI_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
     NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    if (info != 0)
+      fail(NSMI_ERR_SOLVER, std::string(f.tri ? "rocsolver_dstedc" : "rocsolver_dsyevd") + " returned info = " +
+                                std::to_string(info));
     fact_spectrum(f, wh);
   }
 }
@@ -318,7 +315,8 @@ void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
   for (FactCore* f : fs)
     if (f->jacobi) jobs.push_back(JacobiJob{f->rho->as<double>(), f->wdev->as<double>(), int(f->iso)});
   if (jobs.empty()) return;
-  jacobi_eig_batched(ctx, jobs);
+  unsigned unconverged = 0;
+  jacobi_eig_batched(ctx, jobs, &unconverged);
   std::vector<std::vector<double>> wh;
   wh.reserve(jobs.size());
   for (FactCore* f : fs)
@@ -328,6 +326,7 @@ void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
                               ctx->stream));
     }
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  if (unconverged) fail(NSMI_ERR_SOLVER, "jacobi: a Gram matrix did not converge within the sweep limit");
   size_t j = 0;
   for (FactCore* f : fs)
     if (f->jacobi) {
@@ -392,8 +391,8 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
       NSMI_HIP(hipMemcpyAsync(U, Ut.p, sizeof(double) * k * nc, hipMemcpyDeviceToDevice, ctx->stream));
       gemm_f64(ctx, gdesc(m, k, nc, false, true), 1.0, f.M, Ut.as<double>(), 0.0, C, kRoleFact);
     }
-    // the host order vector must outlive the copy above
-    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    // f.order (the source of the index copy above) lives as long as f, which the
+    // entry points keep until their closing stream synchronisation
   }
 }
 

@@ -240,14 +240,15 @@ int sytrd_coop_groups(int n);
 bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau);
 // Batched Jacobi eigensolver for small symmetric matrices (one workgroup each, LDS
 // resident): eigenvectors overwrite a (column-major, dsyevd layout), eigenvalues
-// (unsorted) go to w.  Synchronises.
+// (unsorted) go to w.  Enqueues only: *unconverged (host memory) receives the
+// not-converged flag once the stream reaches it; the caller synchronises.
 constexpr int kJacobiMax = 40;
 struct JacobiJob {
   double* a;
   double* w;
   int n;
 };
-void jacobi_eig_batched(nsmi_ctx* ctx, const std::vector<JacobiJob>& jobs);
+void jacobi_eig_batched(nsmi_ctx* ctx, const std::vector<JacobiJob>& jobs, unsigned* unconverged);
 
 // complex factorize / QR (zfactorize.hip); the truncation rule is passed in
 void zfactorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int rows_trailing, const nsmi_trunc* trunc,

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import itertools
+import math
 
 import numpy as np
 
@@ -67,27 +68,42 @@ class QNTensor:
     """A block-sparse tensor: ``keys`` (tuples of sector ids) in storage order, each
     block dense with the sector dims, all in one flat device buffer."""
 
-    def __init__(self, ctx: Context, indices, dirs, flux=0, keys=None, data: DeviceTensor | None = None):
+    def __init__(self, ctx: Context, indices, dirs, flux=0, keys=None, data: DeviceTensor | None = None,
+                 zero=True):
         self.ctx = ctx
         self.indices = list(indices)
         self.dirs = list(dirs)
         self.flux = int(flux)
         if keys is None:
-            keys = [k for k in itertools.product(*[range(len(ix)) for ix in self.indices])
-                    if sum(d * ix.q[i] for d, ix, i in zip(self.dirs, self.indices, k)) == self.flux]
+            keys = self._allowed_keys()
         self.keys = [tuple(k) for k in keys]
         self.pos = {k: i for i, k in enumerate(self.keys)}
         self.shapes = [tuple(ix.n[i] for ix, i in zip(self.indices, k)) for k in self.keys]
-        sizes = [int(np.prod(s)) for s in self.shapes]
-        self.offs = [int(x) for x in np.concatenate([[0], np.cumsum(sizes)])[:-1]] if sizes else []
+        sizes = [math.prod(s) for s in self.shapes]
+        self.offs = [0] + list(itertools.accumulate(sizes))[:-1] if sizes else []
         self.size = int(sum(sizes))
-        if data is None:
-            data = ctx.upload(np.zeros(max(self.size, 1)))
+        if data is None:   # zeros, or an unset buffer for an output every block of which gets written
+            data = ctx.upload(np.zeros(max(self.size, 1))) if zero else ctx.empty((max(self.size, 1),))
         self.data = data
+
+    def _allowed_keys(self):
+        """Every charge-allowed key in lexicographic order: the leading indices run
+        over all their sectors and the last one takes the sectors of the charge
+        that balances the flux."""
+        *lead, last = self.indices
+        *dl, d_last = self.dirs
+        keys = []
+        for k in itertools.product(*[range(len(ix)) for ix in lead]):
+            q = self.flux - sum(d * ix.q[i] for d, ix, i in zip(dl, lead, k))
+            if q % d_last:
+                continue
+            for j in last._by_q.get(q // d_last, ()):
+                keys.append(k + (j,))
+        return keys
 
     def rows_cols(self, key):
         s = self.shapes[self.pos[key]]
-        return s[0], int(np.prod(s[1:]))
+        return s[0], math.prod(s[1:])
 
     def block_off(self, key):
         return self.offs[self.pos[key]]
@@ -196,19 +212,21 @@ class _Program:
         return 2 + len(self.operands) + j
 
     def build(self, ctx: Context, x_elems, y_elems, precision="fp64"):
-        def arr(rows, w, dt=C.c_int64):
-            flat = [v for r in rows for v in r] if rows else [0] * w
-            return (dt * max(len(flat), 1))(*flat)
+        keep = []
+
+        def arr(rows, w, dt=np.int64, ct=C.c_int64):   # rows of w entries as one contiguous array
+            a = np.asarray(rows, dtype=dt).reshape(-1) if len(rows) else np.zeros(w, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(ct))
 
         ops = (C.c_void_p * max(len(self.operands), 1))(*[o.h.value for o in self.operands])
         scr = (C.c_int64 * max(len(self.scratch), 1))(*self.scratch)
         h = C.c_void_p()
         ctx.check(ctx.lib.nsmi_bsop_create(
             ctx.h, len(self.operands), ops, len(self.scratch), scr, int(x_elems), int(y_elems), len(self.steps),
-            arr(self.steps, 6), arr(self.gitems, 7), arr(self.segs, 7),
-            (C.c_double * max(len(self.alpha), 1))(*self.alpha), arr(self.litems, 5), arr(self.terms, 2),
-            (C.c_double * max(len(self.coeff), 1))(*self.coeff), arr(self.citems, 8), kernels._prec(precision),
-            C.byref(h)))
+            arr(self.steps, 6), arr(self.gitems, 7), arr(self.segs, 7), arr(self.alpha, 1, np.float64, C.c_double),
+            arr(self.litems, 5), arr(self.terms, 2), arr(self.coeff, 1, np.float64, C.c_double),
+            arr(self.citems, 8), kernels._prec(precision), C.byref(h)))
         return BlockProgram(ctx, h, self.operands, precision)
 
 
@@ -415,7 +433,7 @@ def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
     Y = W-mix (LINCOMB), E' = A^T.Y (GEMM with segments over (a', s'))."""
     l, site, r = A.indices
     mw_r = QNIndex([(int(q), 1) for q in cr])
-    out = QNTensor(ctx, [r, mw_r, r], [1, -1, -1], 0)
+    out = QNTensor(ctx, [r, mw_r, r], [1, -1, -1], 0, zero=False)
     mix = _mix_left(E, A, W, cl)
     p = _Program()
     xE, xA = p.operand(E.data), 0
@@ -439,7 +457,7 @@ def env_right(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
     X = A.E^T (GEMM), Y = W-mix (LINCOMB), E' = A.Y^T (GEMM, segments over (s', b'))."""
     l, site, r = A.indices
     mw_l = QNIndex([(int(q), 1) for q in cl])
-    out = QNTensor(ctx, [l, mw_l, l], [1, -1, -1], 0)
+    out = QNTensor(ctx, [l, mw_l, l], [1, -1, -1], 0, zero=False)
     mix = _mix_right(E, A, W, cr)
     p = _Program()
     xE, xA = p.operand(E.data), 0
@@ -464,7 +482,7 @@ def theta(A: QNTensor, B: QNTensor, ctx) -> QNTensor:
     is zero)."""
     l, s1i, m = A.indices
     _, s2i, r = B.indices
-    th = QNTensor(ctx, [l, s1i, s2i, r], [1, 1, 1, -1], 0)
+    th = QNTensor(ctx, [l, s1i, s2i, r], [1, 1, 1, -1], 0, zero=False)
     p = _Program()
     xB = p.operand(B.data)
     items = []
@@ -557,7 +575,7 @@ def factorize(th: QNTensor, rows_trailing: bool, *, cutoff=0.0, maxdim=None, min
         M_off.append(tot)
         tot += R_[-1] * C_[-1]
     qi = {q: i for i, q in enumerate(sectors)}
-    Mall = ctx.upload(np.zeros(max(tot, 1)))
+    Mall = ctx.empty((max(tot, 1),))   # every (row, column) block of a sector is a block of theta
     p = _Program()
     yM = p.operand(Mall)
     items = []
@@ -570,8 +588,8 @@ def factorize(th: QNTensor, rows_trailing: bool, *, cutoff=0.0, maxdim=None, min
     Uall, Call, ks, ti = _factorize_sectors(ctx, Mall, M_off, R_, C_, rows_trailing, cutoff, maxdim, mindim,
                                             which_decomp)
     mid = QNIndex([(q, k) for q, k in zip(sectors, ks) if k > 0])
-    left = QNTensor(ctx, [l, s1i, mid], [1, 1, -1], 0)
-    right = QNTensor(ctx, [mid, s2i, r], [1, 1, -1], 0)
+    left = QNTensor(ctx, [l, s1i, mid], [1, 1, -1], 0, zero=False)
+    right = QNTensor(ctx, [mid, s2i, r], [1, 1, -1], 0, zero=False)
     # offsets of U_q / C_q in the concatenated outputs
     uoff, coff, uo, cof = {}, {}, 0, 0
     for i, q in enumerate(sectors):
@@ -686,7 +704,8 @@ def extract(psi: QNMPS, region):
 def heff_map(P: QNProjTTN, psi: QNMPS, precision="fp64"):
     lo, hi = min(P.pos), max(P.pos)
     L, R = P.left_right(psi)
-    th = QNTensor(psi.ctx, [psi.links[lo], psi.site, psi.site, psi.links[hi + 1]], [1, 1, 1, -1], 0)
+    th = QNTensor(psi.ctx, [psi.links[lo], psi.site, psi.site, psi.links[hi + 1]], [1, 1, 1, -1], 0,
+                   zero=False)   # the block structure only
     return heff2(L, P.H.W[lo], P.H.W[hi], R, th, P.H.c[lo], P.H.c[hi + 1], psi.ctx, precision)
 
 
@@ -868,13 +887,13 @@ def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north
     new_q = sorted(set(a.q) | set(kq))
     ax = QNIndex([(q, na.get(q, 0) + kq.get(q, 0)) for q in new_q])
     if left:
-        newA = QNTensor(ctx, [lx, site, ax], [1, 1, -1], 0)
-        newC = QNTensor(ctx, [ax] + C.indices[1:], [1, 1, -1], 0)
-        newT = QNTensor(ctx, [ax] + local_state.indices[1:], [1, 1, 1, -1], 0)
+        newA = QNTensor(ctx, [lx, site, ax], [1, 1, -1], 0, zero=False)
+        newC = QNTensor(ctx, [ax] + C.indices[1:], [1, 1, -1], 0, zero=False)
+        newT = QNTensor(ctx, [ax] + local_state.indices[1:], [1, 1, 1, -1], 0, zero=False)
     else:
-        newA = QNTensor(ctx, [ax, site, rx], [1, 1, -1], 0)
-        newC = QNTensor(ctx, C.indices[:2] + [ax], [1, 1, -1], 0)
-        newT = QNTensor(ctx, local_state.indices[:3] + [ax], [1, 1, 1, -1], 0)
+        newA = QNTensor(ctx, [ax, site, rx], [1, 1, -1], 0, zero=False)
+        newC = QNTensor(ctx, C.indices[:2] + [ax], [1, 1, -1], 0, zero=False)
+        newT = QNTensor(ctx, local_state.indices[:3] + [ax], [1, 1, 1, -1], 0, zero=False)
     both = [q for q in kq if na[q] > 0]            # charges where U meets A
     c_off, t = {}, 0
     for q in both:
@@ -978,12 +997,12 @@ def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north
             sg = []
             shp = T_new.shapes[T_new.pos[key]]
             if left:
-                w = int(np.prod(shp[1:]))
+                w = math.prod(shp[1:])
                 if n_old and T_old.has(okey):
                     sg.append((sEx, ex_off[q], n_old, x_old, T_old.block_off(okey), w, n_old, 1.0))
                 items.append((y_new, T_new.block_off(key), w, nax, w, sg))
             else:
-                h = int(np.prod(shp[:-1]))
+                h = math.prod(shp[:-1])
                 if n_old and T_old.has(okey):
                     sg.append((x_old, T_old.block_off(okey), n_old, sEx, ex_off[q], n_old, n_old, 1.0))
                 items.append((y_new, T_new.block_off(key), nax, h, nax, sg))
