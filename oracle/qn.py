@@ -266,8 +266,52 @@ def densitymatrix_expand_qn(mps, links, qs, cW, W, envs, prev_pos, region, local
     else:
         mps[p] = Ax.reshape(n1, n2, K).transpose(2, 0, 1).copy()
         mps[v] = np.tensordot(mps[v], expander, axes=([2], [1]))
-        local = np.tensordot(local, expander, axes=([3], [1]))
+        local = np.tensordot(local, expander, axes=([local.ndim - 1], [1]))
     return local, [p, v]
+
+
+def qn_qr_toward(mps, links, qs, x, y):
+    """One step of the orthogonalize walk (extracter.jl:6) on a QN chain, moving the
+    gauge centre from site x to its neighbour y: ITensors qr of a QN tensor is
+    block-wise, each charge of the bond between x and y decomposed on its own
+    (linalg.qr_pos gauge, thin: min(rows, cols) vectors per charge); site x
+    becomes the isometry, y absorbs R."""
+    A = mps[x]
+    dl, d, dr = A.shape
+    if y == x + 1:
+        M = A.reshape(dl * d, dr)                                        # rows (l, s), columns r
+        qrow = (links[x].charges()[:, None] + qs[None, :]).reshape(-1)
+        bond = x + 1
+    else:
+        M = A.reshape(dl, d * dr).T                                      # rows (s, r), columns l
+        qrow = (-qs[:, None] + links[x + 1].charges()[None, :]).reshape(-1)
+        bond = x
+    qcol = links[bond].charges()
+    sec, Qs, Rs = [], [], []
+    for q, _ in links[bond].sectors:
+        ri, ci = np.nonzero(qrow == q)[0], np.nonzero(qcol == q)[0]
+        if len(ri) == 0:
+            continue
+        Qq, Rq = linalg.qr_pos(M[np.ix_(ri, ci)])
+        sec.append((int(q), Qq.shape[1]))
+        Qs.append((ri, Qq))
+        Rs.append((ci, Rq))
+    K = sum(k for _, k in sec)
+    Q = np.zeros((M.shape[0], K))
+    R = np.zeros((K, M.shape[1]))
+    o = 0
+    for (ri, Qq), (ci, Rq) in zip(Qs, Rs):
+        k = Qq.shape[1]
+        Q[ri, o:o + k] = Qq
+        R[o:o + k, ci] = Rq
+        o += k
+    links[bond] = Link(sec)
+    if y == x + 1:
+        mps[x] = Q.reshape(dl, d, K)
+        mps[y] = np.tensordot(R, mps[y], axes=([1], [0]))
+    else:
+        mps[x] = Q.T.reshape(K, d, dr).copy()
+        mps[y] = np.tensordot(mps[y], R.T, axes=([2], [0]))
 
 
 @dataclass
@@ -279,12 +323,18 @@ class QNResult:
 
 
 def qn_dmrg_chain(W, states, d, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, krylovdim=3, tol=1e-14,
-                  which_decomp=None, subspace_algorithm=None, expansion_factor=1.5, max_expand=None, north_pass=1):
-    """2-site QN DMRG from a product state (basis state states[i] on site i) over
-    the Euler-tour plan from the last site, with the densitymatrix subspace
-    expansion in the extracter when asked (extracter.jl:10-12, the operator's
-    previous position, the inserter's truncation); log rows (region, eigenvalue,
-    kept, truncerr, new link sectors)."""
+                  which_decomp=None, subspace_algorithm=None, expansion_factor=1.5, max_expand=None, north_pass=1,
+                  nsites=2):
+    """QN DMRG from a product state (basis state states[i] on site i) over the
+    Euler-tour plan from the last site (edges for nsites=2, vertices for
+    nsites=1), with the densitymatrix subspace expansion in the extracter when
+    asked (extracter.jl:10-12, the operator's previous position, the inserter's
+    truncation); log rows (region, eigenvalue, kept, truncerr, new link sectors),
+    kept and truncerr None for 1-site regions (inserter.jl stores the tensor)."""
+    if nsites == 1:
+        return _qn_dmrg_1site(W, states, d, nsweeps=nsweeps, maxdim=maxdim, cutoff=cutoff, mindim=mindim,
+                              krylovdim=krylovdim, tol=tol, subspace_algorithm=subspace_algorithm,
+                              expansion_factor=expansion_factor, max_expand=max_expand, north_pass=north_pass)
     N = len(W)
     qs = spin_charges(d)
     links = [Link([(0, 1)])]
@@ -329,4 +379,67 @@ def qn_dmrg_chain(W, states, d, *, nsweeps, maxdim=None, cutoff=0.0, mindim=1, k
             envs.site_changed(lo)
             envs.site_changed(hi)
             log.append(((u, v), E, sum(k for _, k in mid.sectors), te, list(mid.sectors)))
+    return QNResult(E, mps, links, log)
+
+
+def _product_chain(states, d):
+    qs = spin_charges(d)
+    links = [Link([(0, 1)])]
+    q = 0
+    mps = []
+    for s in states:
+        q += qs[s]
+        links.append(Link([(int(q), 1)]))
+        a = np.zeros((1, d, 1))
+        a[0, s, 0] = 1.0
+        mps.append(a)
+    return mps, links
+
+
+def _qn_dmrg_1site(W, states, d, *, nsweeps, maxdim, cutoff, mindim, krylovdim, tol, subspace_algorithm,
+                   expansion_factor, max_expand, north_pass):
+    """1-site QN DMRG: per region the QR walk brings the gauge centre to the site,
+    the expansion (if asked) grows the link to the previous position, Lanczos on
+    the 1-site H_eff, and the site stores the Ritz vector (inserter.jl:17-19)."""
+    from .chain import heff1_apply
+    from .plans import chain_graph, euler_tour_vertices
+    N = len(W)
+    qs = spin_charges(d)
+    if subspace_algorithm not in (None, "densitymatrix"):
+        raise NotImplementedError("Subspace expansion (subspace_expand!) not defined for requested combination "
+                                  "of subspace_algorithm and problem types")
+    cW = mpo_link_charges(W, qs)
+    mps, links = _product_chain(states, d)
+    envs = ChainEnvs(W)
+    log, E = [], None
+    pos = list(range(N))
+    centre = N - 1                       # a product state is orthonormal about any site
+    plan = euler_tour_vertices(*chain_graph(N), N - 1)
+    for sweep in range(1, nsweeps + 1):
+        cut = get_or_last(cutoff, sweep)
+        mxd = get_or_last(maxdim, sweep) if maxdim is not None else None
+        mnd = get_or_last(mindim, sweep)
+        for v in plan:
+            while centre != v:
+                nxt = centre + (1 if v > centre else -1)
+                qn_qr_toward(mps, links, qs, centre, nxt)
+                envs.site_changed(centre)
+                envs.site_changed(nxt)
+                centre = nxt
+            th = mps[v]
+            if subspace_algorithm == "densitymatrix":
+                th, changed = densitymatrix_expand_qn(mps, links, qs, cW, W, envs, pos, (v, v), th, cutoff=cut,
+                                                      maxdim=mxd, mindim=mnd, expansion_factor=expansion_factor,
+                                                      max_expand=max_expand, north_pass=north_pass)
+                for c in changed:
+                    envs.site_changed(c)
+            pos = [v]
+            L = envs.get(mps, v - 1, v)
+            R = envs.get(mps, v + 1, v)
+            shape = th.shape
+            E, x, _ = linalg.lanczos_eigsolve(lambda y: heff1_apply(L, W[v], R, y.reshape(shape)).reshape(-1),
+                                              th.reshape(-1), krylovdim=krylovdim, tol=tol)
+            mps[v] = x.reshape(shape)
+            envs.site_changed(v)
+            log.append(((v,), E, None, None, None))
     return QNResult(E, mps, links, log)

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 import scipy.linalg as sl
 
-from oracle import chain, models
+from oracle import chain, linalg, models
 from oracle import qn as oqn
 
 
@@ -152,12 +152,14 @@ def test_qn_expansion_equals_dense_expansion(factor, maxdim):
         assert abs(a[1] - b[1]) < 1e-10 * abs(b[1])
 
 
+@pytest.mark.parametrize("nsites", [2, 1])
 @pytest.mark.parametrize("left", [True, False])
-def test_qn_expansion_step_invariants(left):
-    """One expansion step on a converged-ish state: [A | U] is an isometry, the
-    two-site product A.theta is unchanged (expander = [A|U]^T A keeps the state),
-    each link charge gains at most its own new vectors, and every tensor keeps zero
-    weight outside the charge-allowed blocks."""
+def test_qn_expansion_step_invariants(left, nsites):
+    """One expansion step on a converged-ish state (2-site region, or 1-site region
+    with the local tensor the site itself): [A | U] is an isometry, the product
+    A.theta is unchanged (expander = [A|U]^T A keeps the state), each link charge
+    gains at most its own new vectors, and every tensor keeps zero weight outside
+    the charge-allowed blocks."""
     N = 8
     W = models.heisenberg_mpo(N, d=3)
     qs = oqn.spin_charges(3)
@@ -166,6 +168,8 @@ def test_qn_expansion_step_invariants(left):
     cW = oqn.mpo_link_charges(W, qs)
     envs = chain.ChainEnvs(W)
     lo, hi = (4, 5) if left else (2, 3)
+    if nsites == 1:
+        lo, hi = (lo, lo) if left else (hi, hi)
     p = lo - 1 if left else hi + 1
     ia = p + 1 if left else p
     # put the gauge centre on the region, as the extracter does: p is an isometry toward it
@@ -182,16 +186,18 @@ def test_qn_expansion_step_invariants(left):
         if Q.shape[1] == B.shape[0]:
             mps[i] = Q.T.reshape(mps[i].shape)
             mps[i - 1] = np.tensordot(mps[i - 1], Rm.T, axes=([2], [0]))
-    th = chain.theta2(mps[lo], mps[hi])
+    th = chain.theta2(mps[lo], mps[hi]) if nsites == 2 else mps[lo].copy()
     A0, old = mps[p].copy(), links[ia]
-    before = (np.tensordot(A0, th, axes=([2], [0])) if left else np.tensordot(th, A0, axes=([3], [0])))
-    th2, changed = oqn.densitymatrix_expand_qn(mps, links, qs, cW, W, envs, [lo - 1, lo] if left else [hi, hi + 1],
-                                               (lo, hi), th, cutoff=1e-12, maxdim=64, mindim=1)
+    last = th.ndim - 1
+    before = (np.tensordot(A0, th, axes=([2], [0])) if left else np.tensordot(th, A0, axes=([last], [0])))
+    prev = ([lo - 1, lo] if left else [hi, hi + 1]) if nsites == 2 else [p]
+    th2, changed = oqn.densitymatrix_expand_qn(mps, links, qs, cW, W, envs, prev, (lo, hi), th, cutoff=1e-12,
+                                               maxdim=64, mindim=1)
     assert sorted(changed) == sorted([p, lo if left else hi])
     A1 = mps[p]
     Am = A1.reshape(-1, A1.shape[2]) if left else A1.reshape(A1.shape[0], -1).T
     np.testing.assert_allclose(Am.T @ Am, np.eye(Am.shape[1]), atol=1e-10)
-    after = (np.tensordot(A1, th2, axes=([2], [0])) if left else np.tensordot(th2, A1, axes=([3], [0])))
+    after = (np.tensordot(A1, th2, axes=([2], [0])) if left else np.tensordot(th2, A1, axes=([last], [0])))
     np.testing.assert_allclose(after, before, atol=1e-10)
     new = dict(links[ia].sectors)
     assert sum(new.values()) > sum(n for _, n in old.sectors)
@@ -202,3 +208,64 @@ def test_qn_expansion_step_invariants(left):
     ql, qa = (links[p].charges(), links[p + 1].charges())
     allowed = ql[:, None, None] + qs[None, :, None] == qa[None, None, :]
     assert np.max(np.abs(A1[~allowed]), initial=0.0) < 1e-14
+
+
+@pytest.mark.parametrize("right", [True, False])
+def test_qn_qr_step(right):
+    """The block-wise QR of the orthogonalize walk (ITensors qr of a QN tensor):
+    the two-site product is unchanged, the new site tensor is an isometry toward
+    the moved centre, each bond charge keeps min(rows, cols) vectors, and with
+    every sector tall (full column rank) Q and R equal the dense qr_pos ones."""
+    rng = np.random.default_rng(3)
+    d = 3
+    qs = oqn.spin_charges(d)
+    ll = oqn.Link([(-2, 2), (0, 3), (2, 2)])
+    lm = oqn.Link([(-4, 1), (-2, 2), (0, 3), (2, 2), (4, 1)])
+    lr = oqn.Link([(-2, 3), (0, 4), (2, 3)])
+
+    def rand_site(a, b):
+        qa, qb = a.charges(), b.charges()
+        allowed = qa[:, None, None] + qs[None, :, None] == qb[None, None, :]
+        return np.where(allowed, rng.standard_normal(allowed.shape), 0.0)
+
+    mps = [rand_site(ll, lm), rand_site(lm, lr)]
+    links = [ll, lm, lr]
+    before = np.tensordot(mps[0], mps[1], axes=([2], [0]))
+    x, y = (0, 1) if right else (1, 0)
+    A0 = mps[x].copy()
+    oqn.qn_qr_toward(mps, links, qs, x, y)
+    after = np.tensordot(mps[0], mps[1], axes=([2], [0]))
+    np.testing.assert_allclose(after, before, atol=1e-12)
+    A = mps[x]
+    M = A.reshape(-1, A.shape[2]) if right else A.reshape(A.shape[0], -1).T
+    np.testing.assert_allclose(M.T @ M, np.eye(M.shape[1]), atol=1e-12)
+    # rows per bond charge vs the old bond's sector sizes
+    if right:
+        qrow = (ll.charges()[:, None] + qs[None, :]).reshape(-1)
+    else:
+        qrow = (-qs[:, None] + lr.charges()[None, :]).reshape(-1)
+    expect = [(q, min(int(np.sum(qrow == q)), n)) for q, n in lm.sectors if np.sum(qrow == q) > 0]
+    assert links[1].sectors == expect
+    if all(int(np.sum(qrow == q)) >= n for q, n in lm.sectors):
+        M0 = A0.reshape(-1, A0.shape[2]) if right else A0.reshape(A0.shape[0], -1).T
+        Qd, _ = linalg.qr_pos(M0)
+        np.testing.assert_allclose(M, Qd, atol=1e-12)
+
+
+def test_qn_1site_densitymatrix_reaches_ground_state():
+    """1-site QN DMRG with the densitymatrix expansion (the reference README's
+    1-site form, max_expand 4) from the Neel state: the expansion grows the links
+    from dimension 1, the energy reaches the spin-1 N=10 ED value within the
+    reference's bar for 1-site DMRG with expansion (1e-5, test_tree_dmrg.jl:67),
+    and the last Ritz value equals <psi|H|psi>.  (The block-wise QR keeps at most
+    min(rows, cols) vectors per charge, so the QN sweep has fewer directions to
+    rotate into than the dense one, which keeps rank-deficient columns; here it
+    stops at 1.2e-6 above the ED energy where the dense sweep reaches it.)"""
+    N = 10
+    W = models.heisenberg_mpo(N, d=3)
+    q = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsites=1, nsweeps=12, maxdim=[10, 40, 80, 160], cutoff=1e-12,
+                          subspace_algorithm="densitymatrix", max_expand=4)
+    assert abs(q.energy - _ed("heisenberg_s1_chain", "10")) < 1e-5
+    assert abs(chain.mpo_expectation(W, q.mps) - q.energy) < 1e-10
+    assert max(sum(n for _, n in lk.sectors) for lk in q.links) > 1
+    assert all(q_ == 0 for q_, _ in q.links[-1].sectors)
