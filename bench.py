@@ -369,9 +369,7 @@ def readme_workload(args):
         return eigsolve_problem(prob, nsweeps=nsweeps, nsites=nsites, extracter_kwargs=ek,
                                 inserter_kwargs=dict(trunc=trunc))
 
-    forms = ["2site_dense", "2site_qn", "1site_dense_expand"]
-    if getattr(qn, "SERVES_1SITE", False):
-        forms.append("1site_qn_expand")
+    forms = ["2site_dense", "2site_qn", "1site_dense_expand", "1site_qn_expand"]
     res = {}
     for form in forms:
         solve(form, 1)

@@ -545,6 +545,30 @@ def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx, precision
     return p.build(ctx, th.size, th.size, precision)
 
 
+def heff1(L: QNTensor, W, R: QNTensor, A: QNTensor, cL, cR, ctx, precision="fp64") -> BlockProgram:
+    """One-site effective Hamiltonian on A's block space (the 1-site regions of
+    operator_map.jl:3-10): the left environment update's first two steps give
+    Y[(a', s', v, b)] = sum W[w, v, s', s] L[a', w, a] x[a, s, b], then
+    y[a', s', b'] = sum_{v, b} Y R[b', v, b]^T (GEMM with segments over the right
+    MPO link)."""
+    l, site, r = A.indices
+    mix = _mix_left(L, A, W, cL)
+    p = _Program()
+    xL, xR = p.operand(L.data), p.operand(R.data)
+    sY = mix.emit(p, xL, 0)
+    items = []
+    for (ap, sp, bp) in A.keys:
+        sg = []
+        for v in range(len(cR)):
+            b = r.find(r.q[bp] - cR[v])
+            if b is None or (ap, sp, v, b) not in mix.Y or not R.has((bp, v, b)):
+                continue
+            sg.append((sY, mix.Y[(ap, sp, v, b)], r.n[b], xR, R.block_off((bp, v, b)), r.n[b], r.n[b], 1.0))
+        items.append((1, A.block_off((ap, sp, bp)), r.n[bp], l.n[ap], r.n[bp], sg))
+    p.gemm(False, True, items)
+    return p.build(ctx, A.size, A.size, precision)
+
+
 def factorize(th: QNTensor, rows_trailing: bool, *, cutoff=0.0, maxdim=None, mindim=1, which_decomp=None, ctx=None):
     """Truncated split of theta[l, s1, s2, r] over the new link between s1 and s2
     (inserter.jl:23 with a QN ITensor): sector q = q_l + q_s1 gathers rows (l, s1)
@@ -691,25 +715,35 @@ class QNProjTTN:
 
 # ------------------------------------------------------------ region step hooks
 def extract(psi: QNMPS, region):
+    """extracter.jl:3-8: the gauge centre walks into the region, then the local
+    tensor is the site (1-site) or the product of the two sites (2-site)."""
     lo, hi = min(region), max(region)
-    if len(region) != 2 or hi != lo + 1:
-        raise NotImplementedError("QN chains: 2-site regions on neighbouring sites are served")
-    if psi.ortho_region is None or not all(lo <= v <= hi for v in psi.ortho_region):
-        raise NotImplementedError("QN chains: the orthogonality centre must already lie in the region "
-                                  "(sweeps from product states keep it there)")
+    if len(region) not in (1, 2) or hi - lo != len(region) - 1:
+        raise NotImplementedError("QN chains: 1-site regions and 2-site regions on neighbouring sites are served")
+    orthogonalize(psi, region)
     psi.ortho_region = list(region)
+    if len(region) == 1:
+        return psi[lo].copy()
     return theta(psi[lo], psi[hi], psi.ctx)
 
 
 def heff_map(P: QNProjTTN, psi: QNMPS, precision="fp64"):
     lo, hi = min(P.pos), max(P.pos)
     L, R = P.left_right(psi)
+    if lo == hi:   # the block structure of the site, which the local tensor copies
+        A = QNTensor(psi.ctx, psi[lo].indices, psi[lo].dirs, 0, keys=psi[lo].keys, zero=False)
+        return heff1(L, P.H.W[lo], R, A, P.H.c[lo], P.H.c[lo + 1], psi.ctx, precision)
     th = QNTensor(psi.ctx, [psi.links[lo], psi.site, psi.site, psi.links[hi + 1]], [1, 1, 1, -1], 0,
                    zero=False)   # the block structure only
     return heff2(L, P.H.W[lo], P.H.W[hi], R, th, P.H.c[lo], P.H.c[hi + 1], psi.ctx, precision)
 
 
 def insert(psi: QNMPS, region, th: QNTensor, trunc, which_decomp=None):
+    """inserter.jl:3-33: a 1-site region stores the tensor (no truncation); a
+    2-site region is split by the sector factorisation."""
+    if len(region) == 1:
+        psi[region[0]] = th
+        return None
     u, v = region
     lo, hi = min(u, v), max(u, v)
     left, right, mid, info = factorize(th, u != lo, cutoff=trunc["cutoff"], maxdim=trunc["maxdim"],
@@ -718,6 +752,149 @@ def insert(psi: QNMPS, region, th: QNTensor, trunc, which_decomp=None):
     psi[lo] = left
     psi[hi] = right
     return info
+
+
+# ------------------------------------------------------------ orthogonalize walk
+_CHUNK = 24   # per-sector tensors per block program (a program addresses at most 30 buffers)
+
+
+def _run_chunked(ctx, fixed, per_sector, emit):
+    """Runs emit(program, fixed buffer ids, [(sector, buffer id)]) over chunks of
+    the per-sector tensors so that every program stays within its buffer limit."""
+    keys = list(per_sector)
+    for c0 in range(0, max(len(keys), 1), _CHUNK):
+        chunk = keys[c0:c0 + _CHUNK]
+        if not chunk:
+            break
+        p = _Program()
+        fid = [p.operand(t) for t in fixed]
+        sid = [(q, p.operand(per_sector[q])) for q in chunk]
+        emit(p, fid, sid)
+        _run_once(p, ctx, None, None)
+
+
+def qr_toward(psi: QNMPS, x, y):
+    """One step of the orthogonalize walk (extracter.jl:6) on a QN chain: the gauge
+    centre moves from site x to its neighbour y by the QR of a QN tensor, which is
+    block-wise -- each charge of the bond between x and y is decomposed on its own
+    (``nsmi_qr``, thin, non-negative diag(R)), so the bond keeps min(rows, cols)
+    vectors per charge; x becomes the isometry and y absorbs R."""
+    ctx = psi.ctx
+    A, B = psi[x], psi[y]
+    site = psi.site
+    right = y == x + 1
+    bond = x + 1 if right else x
+    bix = psi.links[bond]
+    if right:
+        lx = A.indices[0]
+        groups = _charge_rows(lx, site, 1, 1)           # rows (l, s) per charge
+    else:
+        rx = A.indices[2]
+        groups = _charge_rows(site, rx, -1, 1)          # columns (s, r) per charge
+    secs = [(b, bix.q[b]) for b in range(len(bix)) if bix.q[b] in groups]
+    mats = {}
+    for b, q in secs:
+        n = groups[q][2]
+        mats[q] = ctx.empty((n, bix.n[b]) if right else (bix.n[b], n))
+
+    def gather(p, fid, sid):
+        xA = fid[0]
+        zero, cp = [], []
+        for q, yq in sid:
+            keys, off, n = groups[q]
+            b = bix.find(q)
+            zero.append((yq, 0, n * bix.n[b], []))
+            for (i, j) in keys:
+                key = (i, j, b) if right else (b, i, j)
+                if not A.has(key):
+                    continue
+                if right:      # block l.n[i] x bond.n[b] at row off
+                    cp.append((xA, A.block_off(key), bix.n[b], yq, off[(i, j)] * bix.n[b], bix.n[b], lx.n[i],
+                               bix.n[b]))
+                else:          # block bond.n[b] x r.n[j] at column off
+                    cp.append((xA, A.block_off(key), rx.n[j], yq, off[(i, j)], n, bix.n[b], rx.n[j]))
+        p.lincomb(zero)
+        if cp:
+            p.copy(cp)
+
+    _run_chunked(ctx, [A.data], mats, gather)
+    Qs, Rs, newsec = {}, {}, []
+    for b, q in secs:
+        Q, R = kernels.qr(mats[q], 1, not right)
+        Qs[q], Rs[q] = Q, R
+        newsec.append((q, Q.shape[1] if right else Q.shape[0]))
+    m = QNIndex(newsec)
+    if right:
+        newA = QNTensor(ctx, [lx, site, m], [1, 1, -1], 0, zero=False)
+        newB = QNTensor(ctx, [m] + B.indices[1:], [1, 1, -1], 0, zero=False)
+    else:
+        newA = QNTensor(ctx, [m, site, rx], [1, 1, -1], 0, zero=False)
+        newB = QNTensor(ctx, B.indices[:2] + [m], [1, 1, -1], 0, zero=False)
+
+    def scatter(p, fid, sid):
+        yA = fid[0]
+        cp = []
+        for q, xq in sid:
+            keys, off, n = groups[q]
+            mm = m.find(q)
+            k = m.n[mm]
+            for (i, j) in keys:
+                key = (i, j, mm) if right else (mm, i, j)
+                if right:      # Q rows off .. + l.n[i], k columns
+                    cp.append((xq, off[(i, j)] * k, k, yA, newA.block_off(key), k, lx.n[i], k))
+                else:          # Q (k x n) columns off .. + r.n[j]
+                    cp.append((xq, off[(i, j)], n, yA, newA.block_off(key), rx.n[j], k, rx.n[j]))
+        p.copy(cp)
+
+    _run_chunked(ctx, [newA.data], Qs, scatter)
+
+    def absorb(p, fid, sid):
+        xB, yB = fid
+        items = []
+        for q, xr in sid:
+            mm, b = m.find(q), bix.find(q)
+            k, nb = m.n[mm], bix.n[b]
+            for key in newB.keys:
+                if (key[0] if right else key[2]) != mm:
+                    continue
+                shp = newB.shapes[newB.pos[key]]
+                okey = ((b,) + key[1:]) if right else (key[:2] + (b,))
+                sg = []
+                if right:      # R (k x nb) . B block (nb x w)
+                    w = math.prod(shp[1:])
+                    if B.has(okey):
+                        sg.append((xr, 0, nb, xB, B.block_off(okey), w, nb, 1.0))
+                    items.append((yB, newB.block_off(key), w, k, w, sg))
+                else:          # B block (h x nb) . R (nb x k)
+                    h = math.prod(shp[:-1])
+                    if B.has(okey):
+                        sg.append((xB, B.block_off(okey), nb, xr, 0, k, nb, 1.0))
+                    items.append((yB, newB.block_off(key), k, h, k, sg))
+        if items:
+            p.gemm(False, False, items)
+
+    _run_chunked(ctx, [B.data, newB.data], Rs, absorb)
+    psi.links[bond] = m
+    psi[x] = newA
+    psi[y] = newB
+    psi.ortho_region = [y]
+
+
+def orthogonalize(psi: QNMPS, region):
+    """Walks the gauge centre into the region (extracter.jl:6) by block-wise QR
+    steps; a no-op when it is already there."""
+    lo, hi = min(region), max(region)
+    cur = psi.ortho_region
+    if cur is not None and all(lo <= v <= hi for v in cur):
+        return psi
+    if cur is None or len(cur) != 1:
+        raise NotImplementedError("QN chains: the gauge centre must be one site to walk it")
+    c = cur[0]
+    while not (lo <= c <= hi):
+        nxt = c + 1 if c < lo else c - 1
+        qr_toward(psi, c, nxt)
+        c = nxt
+    return psi
 
 
 # ------------------------------------------------------------ subspace expansion
@@ -889,11 +1066,11 @@ def densitymatrix_expand(problem, local_state: QNTensor, region, *, trunc, north
     if left:
         newA = QNTensor(ctx, [lx, site, ax], [1, 1, -1], 0, zero=False)
         newC = QNTensor(ctx, [ax] + C.indices[1:], [1, 1, -1], 0, zero=False)
-        newT = QNTensor(ctx, [ax] + local_state.indices[1:], [1, 1, 1, -1], 0, zero=False)
+        newT = QNTensor(ctx, [ax] + local_state.indices[1:], local_state.dirs, 0, zero=False)
     else:
         newA = QNTensor(ctx, [ax, site, rx], [1, 1, -1], 0, zero=False)
         newC = QNTensor(ctx, C.indices[:2] + [ax], [1, 1, -1], 0, zero=False)
-        newT = QNTensor(ctx, local_state.indices[:3] + [ax], [1, 1, 1, -1], 0, zero=False)
+        newT = QNTensor(ctx, local_state.indices[:-1] + [ax], local_state.dirs, 0, zero=False)
     both = [q for q in kq if na[q] > 0]            # charges where U meets A
     c_off, t = {}, 0
     for q in both:
@@ -1042,4 +1219,5 @@ def expect_mpo(psi: QNMPS, H: QNMPO) -> float:
 
 
 __all__ = ["QNIndex", "QNTensor", "QNMPO", "QNMPS", "QNProjTTN", "spin_site_index", "qn_product_mps", "theta",
-           "env_left", "env_right", "heff2", "factorize", "expect_mpo", "BlockProgram", "densitymatrix_expand"]
+           "env_left", "env_right", "heff1", "heff2", "factorize", "expect_mpo", "BlockProgram", "densitymatrix_expand",
+           "qr_toward", "orthogonalize"]

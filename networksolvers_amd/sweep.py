@@ -280,8 +280,8 @@ def inserter(problem, local_tensor, region_iter, *, normalize=False, set_orthogo
     psi = state(problem)
     info = None
     if isinstance(psi, qn.QNMPS):
-        if len(region) != 2:
-            raise NotImplementedError("QN chains: 2-site regions are served")
+        if len(region) > 2:
+            raise ValueError(f"Region of length {len(region)} not currently supported")
         info = qn.insert(psi, list(region), local_tensor, t, which_decomp)
     elif isinstance(psi, tree.TreeTTN):
         if len(region) > 2:

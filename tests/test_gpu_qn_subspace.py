@@ -163,3 +163,74 @@ def test_qn_densitymatrix_reference_example(ctx):
     assert abs(E - ref.energy) < 1e-10 * abs(E)
     assert abs(ns.expect_mpo(psi, H) - E) < 1e-9
     assert max(psi.linkdims()) > 99        # plain 2-site DMRG stops at 99 on this run
+
+
+@pytest.mark.parametrize("right", [True, False])
+def test_qn_qr_step_vs_oracle(ctx, right):
+    """The block-wise QR step of the walk equals the oracle's (same bond sectors,
+    site and neighbour tensors element-wise: the thin QR with non-negative
+    diag(R) is unique for full-rank sectors)."""
+    from networksolvers_amd import qn
+    N = 8
+    W = _disordered_xxz_s1(N, 11)
+    qs = oqn.spin_charges(3)
+    res = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsweeps=2, maxdim=[6, 12], cutoff=1e-10)
+    x, y = (3, 4) if right else (4, 3)
+    o_mps, o_links = [m.copy() for m in res.mps], list(res.links)
+    oqn.qn_qr_toward(o_mps, o_links, qs, x, y)
+    psi, site = _device_state(ctx, res.mps, res.links)
+    psi.ortho_region = [x]
+    qn.qr_toward(psi, x, y)
+    bond = x + 1 if right else x
+    assert list(zip(psi.links[bond].q, psi.links[bond].n)) == [tuple(t) for t in o_links[bond].sectors]
+    np.testing.assert_allclose(psi[x].to_dense(), o_mps[x], atol=1e-12)
+    np.testing.assert_allclose(psi[y].to_dense(), o_mps[y], atol=1e-12)
+    assert psi.ortho_region == [y]
+
+
+def test_qn_heff1_apply(ctx):
+    """The 1-site QN H_eff equals the dense contraction of the same tensors."""
+    from networksolvers_amd import qn
+    N = 8
+    W = _disordered_xxz_s1(N, 13)
+    res = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsweeps=2, maxdim=[6, 12], cutoff=1e-10)
+    psi, site = _device_state(ctx, res.mps, res.links)
+    H = qn.QNMPO(ctx, W, site)
+    P = qn.QNProjTTN(H, ctx)
+    v = 4
+    P.position(psi, [v])
+    op = qn.heff_map(P, psi)
+    x = psi[v].copy()
+    y = psi[v].copy()
+    op.run(x.data, y.data)
+    envs = chain.ChainEnvs(W)
+    ref = chain.heff1_apply(envs.get(res.mps, v - 1, v), W[v], envs.get(res.mps, v + 1, v), res.mps[v])
+    assert np.linalg.norm(y.to_dense() - ref) < 1e-12 * np.linalg.norm(ref)
+    op.close()
+
+
+def test_qn_1site_densitymatrix_vs_oracle(ctx):
+    """1-site QN DMRG with the densitymatrix expansion (max_expand 4, the README's
+    1-site form): region by region the device equals the oracle (Ritz value 1e-10
+    relative), and so do the final link sectors."""
+    import networksolvers_amd as ns
+    from networksolvers_amd import qn
+    N = 10
+    W = _disordered_xxz_s1(N, 17)
+    site = qn.spin_site_index(3)
+    prob = ns.EigsolveProblem(state=qn.qn_product_mps(ctx, _neel_s1(N), site),
+                              operator=ns.sweep.projected_operator(qn.QNMPO(ctx, W, site)))
+    trunc = dict(cutoff=1e-8, maxdim=[8, 16, 32])
+    energies = []
+    E, psi = ns.sweep.eigsolve_problem(prob, nsweeps=4, nsites=1, inserter_kwargs=dict(trunc=trunc),
+                                       extracter_kwargs=dict(trunc=trunc, subspace_algorithm="densitymatrix",
+                                                             max_expand=4),
+                                       region_callback=lambda problem, **kw: energies.append(problem.eigenvalue))
+    ref = oqn.qn_dmrg_chain(W, _neel_s1(N), 3, nsites=1, nsweeps=4, maxdim=[8, 16, 32], cutoff=1e-8,
+                            subspace_algorithm="densitymatrix", max_expand=4)
+    assert len(energies) == len(ref.log)
+    for e, r_ in zip(energies, ref.log):
+        assert abs(e - r_[1]) < 1e-10 * abs(r_[1])
+    for i in range(N + 1):
+        assert list(zip(psi.links[i].q, psi.links[i].n)) == [tuple(t) for t in ref.links[i].sectors]
+    assert max(psi.linkdims()) > 1
