@@ -267,6 +267,10 @@ int nsmi_profile_enable(nsmi_ctx* ctx, int on);
 /* kernel classes recorded while profiling is on: bit k selects NSMI_K_* class k
  * (default: all), so a timed run can carry events on the classes it reports only */
 int nsmi_profile_classes(nsmi_ctx* ctx, unsigned mask);
+/* record one launch in `every` of each selected class (default 1: all), which
+ * keeps the event overhead out of a timed run while still sampling its kernels;
+ * counters restart.  No counterpart in the reference (measurement only). */
+int nsmi_profile_sampling(nsmi_ctx* ctx, unsigned every);
 int nsmi_profile_reset(nsmi_ctx* ctx);
 int nsmi_profile_query(nsmi_ctx* ctx, int kernel_class, nsmi_profile_stats* out);
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "nsmi_profile_enable": (C.c_int, [_VP, C.c_int]),
     "nsmi_profile_reset": (C.c_int, [_VP]),
     "nsmi_profile_classes": (C.c_int, [_VP, C.c_uint]),
+    "nsmi_profile_sampling": (C.c_int, [_VP, C.c_uint]),
     "nsmi_profile_query": (C.c_int, [_VP, C.c_int, C.POINTER(ProfileStats)]),
 }
 

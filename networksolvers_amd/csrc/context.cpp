@@ -125,6 +125,7 @@ static hipEvent_t take_event(nsmi_ctx* ctx) {
 ProfScope::ProfScope(nsmi_ctx* c, int k, double f, double by, bool kt)
     : ctx(c), cls(k), flops(f), bytes(by), kernel_timed(kt) {
   if (!ctx->prof_on || k < 0 || k >= 32 || !((ctx->prof_mask >> k) & 1u)) return;
+  if (ctx->prof_seen[k]++ % ctx->prof_every != 0) return;
   a = take_event(ctx);
   b = take_event(ctx);
   if (kernel_timed) {
@@ -404,6 +405,14 @@ int nsmi_profile_classes(nsmi_ctx* ctx, unsigned mask) {
   if (!ctx) return NSMI_ERR_INVALID;
   API_BEGIN
   ctx->prof_mask = mask;
+  API_END(ctx)
+}
+
+int nsmi_profile_sampling(nsmi_ctx* ctx, unsigned every) {
+  if (!ctx || every == 0) return NSMI_ERR_INVALID;
+  API_BEGIN
+  ctx->prof_every = every;
+  for (unsigned& s : ctx->prof_seen) s = 0;
   API_END(ctx)
 }
 

@@ -89,6 +89,8 @@ struct nsmi_ctx {
   // profiling
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // classes recorded while prof_on
+  unsigned prof_every = 1;   // record one scope in prof_every per class
+  unsigned prof_seen[32] = {};
   // events a kernel-timed ProfScope hands to the next NSMI_LAUNCH_TIMED launch
   hipEvent_t kev_start = nullptr, kev_stop = nullptr;
   bool kev_used = false;

@@ -89,6 +89,10 @@ class Context:
         mask = 0xFFFFFFFF if names is None else sum(1 << _lib.K_CLASSES[n] for n in names)
         self.check(self.lib.nsmi_profile_classes(self.h, mask))
 
+    def profile_sampling(self, every: int = 1):
+        """Record one launch in ``every`` per kernel class while profiling is on."""
+        self.check(self.lib.nsmi_profile_sampling(self.h, int(every)))
+
     def profile_stats(self):
         out = {}
         for name, k in _lib.K_CLASSES.items():
