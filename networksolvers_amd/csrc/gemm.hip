@@ -40,11 +40,15 @@ template <typename T> __device__ inline int out_row(int kl, int r);
 template <> __device__ inline int out_row<double>(int kl, int r) { return kl + 4 * r; }
 template <> __device__ inline int out_row<float>(int kl, int r) { return 4 * kl + r; }
 
-template <typename T>
+// T: the MFMA arithmetic type; TBm / TCm: how B and C are stored (a double B is
+// rounded to T as it is staged, a double C takes the T accumulator widened), so a
+// FP32 product can read an FP64 operand and write an FP64 result without
+// conversion passes.
+template <typename T, typename TBm = T, typename TCm = T>
 struct GemmArgs {
   const T* A;
-  const T* B;
-  T* C;
+  const TBm* B;
+  TCm* C;
   int64_t M, N, K, lda, ldb, ldc, sA, sB, sC;
   T alpha, beta;
   int tiles_m, tiles_n;
@@ -59,8 +63,8 @@ __device__ inline int xcd_remap(int wg, int nwg) {
   return (wg % kXcd) * (nwg / kXcd) + wg / kXcd;
 }
 
-template <typename T, int BM, int BN, bool TA, bool TB, int ROLE>
-__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T> g) {
+template <typename T, typename TBm, typename TCm, int BM, int BN, bool TA, bool TB, int ROLE>
+__global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
   constexpr int BK = 16, PAD = 4;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int FM = WTM / 16, FN = WTN / 16;
@@ -77,8 +81,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T> g) {
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   const int64_t bz = blockIdx.z;
   const T* __restrict__ A = g.A + bz * g.sA;
-  const T* __restrict__ B = g.B + bz * g.sB;
-  T* __restrict__ C = g.C + bz * g.sC;
+  const TBm* __restrict__ B = g.B + bz * g.sB;
+  TCm* __restrict__ C = g.C + bz * g.sC;
 
   T ra[EA], rb[EB];
   auto load = [&](int64_t k0) {
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T> g) {
       const int kk = TB ? (e % BK) : (e / BN);
       const int64_t gn = n0 + nn, gk = k0 + kk;
       T v = T(0);
-      if (gn < g.N && gk < g.K) v = TB ? B[gn * g.ldb + gk] : B[gk * g.ldb + gn];
+      if (gn < g.N && gk < g.K) v = T(TB ? B[gn * g.ldb + gk] : B[gk * g.ldb + gn]);
       rb[j] = v;
     }
   };
@@ -159,41 +163,41 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T> g) {
         const int64_t gn = n0 + wn * WTN + j * 16 + ml;
         if (gm < g.M && gn < g.N) {
           T v = g.alpha * acc[i][j][r];
-          if (g.beta != T(0)) v += g.beta * C[gm * g.ldc + gn];
-          C[gm * g.ldc + gn] = v;
+          if (g.beta != T(0)) v += g.beta * T(C[gm * g.ldc + gn]);
+          C[gm * g.ldc + gn] = TCm(v);
         }
       }
 }
 
-template <typename T, int BM, int BN, int ROLE>
-void launch_tile(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T>& a) {
+template <typename T, typename TBm, typename TCm, int BM, int BN, int ROLE>
+void launch_tile(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T, TBm, TCm>& a) {
   a.tiles_m = int((d.M + BM - 1) / BM);
   a.tiles_n = int((d.N + BN - 1) / BN);
   dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n), 1, unsigned(d.batch));
   dim3 block(256);
   if (!d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, false, false, ROLE>), grid, block, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, false, false, ROLE>), grid, block, a);
   else if (!d.transA && d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, false, true, ROLE>), grid, block, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, false, true, ROLE>), grid, block, a);
   else if (d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, true, false, ROLE>), grid, block, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, true, false, ROLE>), grid, block, a);
   else
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, BM, BN, true, true, ROLE>), grid, block, a);
+    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, true, true, ROLE>), grid, block, a);
   NSMI_LAUNCH_CHECK();
 }
 
-template <typename T, int ROLE>
-void launch_role(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T>& a) {
+template <typename T, typename TBm, typename TCm, int ROLE>
+void launch_role(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T, TBm, TCm>& a) {
   // the largest tile that still gives every CU work: 128x128 when that makes >= 2
   // workgroups per CU, 64x64 when >= 1 per CU, else 32x32 (small bond dimensions)
   const int64_t big = ((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch;
   const int64_t mid = ((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch;
   if (big >= 512)
-    launch_tile<T, 128, 128, ROLE>(ctx, d, a);
+    launch_tile<T, TBm, TCm, 128, 128, ROLE>(ctx, d, a);
   else if (mid >= 256)
-    launch_tile<T, 64, 64, ROLE>(ctx, d, a);
+    launch_tile<T, TBm, TCm, 64, 64, ROLE>(ctx, d, a);
   else
-    launch_tile<T, 32, 32, ROLE>(ctx, d, a);
+    launch_tile<T, TBm, TCm, 32, 32, ROLE>(ctx, d, a);
 }
 
 template <typename T>
@@ -203,11 +207,11 @@ void gemm_generic(nsmi_ctx* ctx, const GemmDesc& d, T alpha, const T* A, const T
   if (d.M == 0 || d.N == 0) return;
   GemmArgs<T> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, alpha, beta, 0, 0};
   switch (role) {
-    case kRoleHeff1: launch_role<T, kRoleHeff1>(ctx, d, a); break;
-    case kRoleHeff2: launch_role<T, kRoleHeff2>(ctx, d, a); break;
-    case kRoleEnv: launch_role<T, kRoleEnv>(ctx, d, a); break;
-    case kRoleTheta: launch_role<T, kRoleTheta>(ctx, d, a); break;
-    default: launch_role<T, kRoleFact>(ctx, d, a); break;
+    case kRoleHeff1: launch_role<T, T, T, kRoleHeff1>(ctx, d, a); break;
+    case kRoleHeff2: launch_role<T, T, T, kRoleHeff2>(ctx, d, a); break;
+    case kRoleEnv: launch_role<T, T, T, kRoleEnv>(ctx, d, a); break;
+    case kRoleTheta: launch_role<T, T, T, kRoleTheta>(ctx, d, a); break;
+    default: launch_role<T, T, T, kRoleFact>(ctx, d, a); break;
   }
 }
 
@@ -225,6 +229,20 @@ void gemm_f64(nsmi_ctx* ctx, const GemmDesc& g, double alpha, const double* A, c
 
 void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, float* C, int role) {
   gemm_generic<float>(ctx, g, 1.0f, A, B, 0.0f, C, role);
+}
+
+void gemm_f32_heff1_bd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const double* B, float* C) {
+  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1 && d.tiles_ok(), "gemm: bad sizes");
+  if (d.M == 0 || d.N == 0) return;
+  GemmArgs<float, double, float> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0};
+  launch_role<float, double, float, kRoleHeff1>(ctx, d, a);
+}
+
+void gemm_f32_heff2_cd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const float* B, double* C) {
+  require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1 && d.tiles_ok(), "gemm: bad sizes");
+  if (d.M == 0 || d.N == 0) return;
+  GemmArgs<float, float, double> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0};
+  launch_role<float, float, double, kRoleHeff2>(ctx, d, a);
 }
 
 }  // namespace nsmi

@@ -169,6 +169,10 @@ void gemm_f64(nsmi_ctx* ctx, const GemmDesc& g, double alpha, const double* A, c
               double beta, double* C, int role);
 // FP32 MFMA on float operands, FP32 result
 void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, float* C, int role);
+// the FP32 H_eff products of a real FP64 vector: GEMM1 reads x (FP64) and GEMM2
+// writes y (FP64) directly, rounding / widening inside the kernel
+void gemm_f32_heff1_bd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const double* B, float* C);
+void gemm_f32_heff2_cd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const float* B, double* C);
 // BF16 MFMA, C[M x N] = A[M x Kp] . B[N x Kp]^T on pre-split bf16 hi (and lo)
 // planes (terms = 1: hi.hi; 3: hi.hi + hi.lo + lo.hi); Kp % 32 == 0 with zero
 // padding; C is FP32 or FP64 (c_double)

@@ -381,20 +381,18 @@ static void heff_apply_impl(nsmi_ctx* ctx, nsmi_heff* op, const double* x, doubl
     return;
   }
   if (prec == NSMI_PREC_FP32) {
+    // x is rounded to FP32 as GEMM1 stages it and y widened to FP64 as GEMM2 stores it
     float* T1 = static_cast<float*>(op->T1);
     float* T3 = static_cast<float*>(op->T3);
-    Scratch xs(ctx, sizeof(float) * op->n), ys(ctx, sizeof(float) * op->n);
-    vec_to_f32(ctx, xs.as<float>(), x, op->n);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 4.0 * (M1 * K1 + K1 * N1 + M1 * N1), true);
-      gemm_f32(ctx, g1, op->L32, xs.as<float>(), T1, kRoleHeff1);
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM1, op->flops_gemm1, 4.0 * (M1 * K1 + M1 * N1) + 8.0 * K1 * N1, true);
+      gemm_f32_heff1_bd(ctx, g1, op->L32, x, T1);
     }
     mix_f32(ctx, T1, T3, op->Mx, op->cl, op->P, op->Q, op->cr, NSMI_K_HEFF_MIX);
     {
-      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 4.0 * (M2 * K2 + K2 * N2 + M2 * N2), true);
-      gemm_f32(ctx, g2, T3, op->R32, ys.as<float>(), kRoleHeff2);
+      ProfScope ps(ctx, NSMI_K_HEFF_GEMM2, op->flops_gemm2, 4.0 * (M2 * K2 + K2 * N2) + 8.0 * M2 * N2, true);
+      gemm_f32_heff2_cd(ctx, g2, T3, op->R32, y);
     }
-    vec_f32_to_f64(ctx, y, ys.as<float>(), op->n);
     return;
   }
   if (prec == NSMI_PREC_BF16X3 || prec == NSMI_PREC_BF16) {
