@@ -132,7 +132,7 @@ __device__ inline void row16_totals(double (&a)[R]) {
   for (int s = 0; s < R; ++s) a[s] += dpp_mov<0x140, 0xF>(a[s]);  // row_mirror
 }
 
-// Row-group layout (the form for one workgroup, n <= 128, where a wave would
+// Row-group layout (the form for one workgroup, n <= 64, where a wave would
 // otherwise own up to 32 rows and their 6-stage reductions): the 256 threads form
 // 16 groups of 16 lanes (DPP rows); group rg of
 // workgroup g owns the matrix rows (r 16 + rg) G + g, r < RS, in registers, its
@@ -437,13 +437,18 @@ struct TriPlan {
   int rows_per_group;
 };
 
-// the smallest instantiation whose padded width covers n: the row-group form on
-// one workgroup, the wave-owned form on several (measured faster there: it needs
-// no workgroup barrier besides the grid barrier)
+// Measured on gfx950 (per column, n = 41 .. 512): the row-group form on one
+// workgroup wins up to n = 64 (2.4-2.6 us); above that the wave-owned form on as
+// many workgroups as the plain-launch cap of 16 allows, i.e. the fewest rows per
+// wave: 3.4 us at n <= 128 (4 rows per wave) against 4.6 us on one workgroup, 3.9-4.3
+// us at n <= 256 (4 rows) against 5.1-5.5 us with 16 rows, 5.2-5.5 us at n <= 384
+// (6 rows) against 5.5-5.8 us with 8.  Each plan uses the narrowest padded width
+// that covers n.
 TriPlan tri_plan(int n) {
   if (n <= 64) return {reinterpret_cast<const void*>(k_sytrd_grp<4, 4, true>), 64};
-  if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_grp<8, 8, true>), 128};
-  if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_wave<4, 16, false>), 64};
+  if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_wave<2, 4, false>), 16};
+  if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_wave<4, 4, false>), 16};
+  if (n <= 384) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 6, false>), 24};
   if (n <= 512) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 8, false>), 32};
   return {reinterpret_cast<const void*>(k_sytrd_wave<16, 4, false>), 16};
 }
