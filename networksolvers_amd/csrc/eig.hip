@@ -442,13 +442,15 @@ struct TriPlan {
 // many workgroups as the plain-launch cap of 16 allows, i.e. the fewest rows per
 // wave: 3.4 us at n <= 128 (4 rows per wave) against 4.6 us on one workgroup, 3.9-4.3
 // us at n <= 256 (4 rows) against 5.1-5.5 us with 16 rows, 5.2-5.5 us at n <= 384
-// (6 rows) against 5.5-5.8 us with 8.  Each plan uses the narrowest padded width
+// (6 rows) against 5.5-5.8 us with 8, 5.7 us at n <= 448 (7 rows) against 5.9 us.
+// Each plan uses the narrowest padded width
 // that covers n.
 TriPlan tri_plan(int n) {
   if (n <= 64) return {reinterpret_cast<const void*>(k_sytrd_grp<4, 4, true>), 64};
   if (n <= 128) return {reinterpret_cast<const void*>(k_sytrd_wave<2, 4, false>), 16};
   if (n <= 256) return {reinterpret_cast<const void*>(k_sytrd_wave<4, 4, false>), 16};
   if (n <= 384) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 6, false>), 24};
+  if (n <= 448) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 7, false>), 28};
   if (n <= 512) return {reinterpret_cast<const void*>(k_sytrd_wave<8, 8, false>), 32};
   return {reinterpret_cast<const void*>(k_sytrd_wave<16, 4, false>), 16};
 }

@@ -229,13 +229,14 @@ def test_factorize(ctx, route, trailing, shape, cutoff, maxdim):
 @pytest.mark.parametrize("m,n,maxdim", [(2, 5, None), (3, 3, None), (31, 40, None), (39, 39, 30), (40, 44, None),
                                          (41, 41, 30), (64, 70, None), (65, 65, None), (96, 100, None),
                                          (128, 130, None), (129, 140, 100), (150, 150, None), (256, 260, 128),
-                                         (257, 300, 128), (384, 390, 200), (385, 400, 200), (512, 520, 256),
+                                         (257, 300, 128), (384, 390, 200), (385, 400, 200), (448, 450, 220), (449, 460, 220),
+                                         (512, 520, 256),
                                          (600, 620, 300), (700, 700, 350), (1100, 1000, 500), (2100, 2100, 700)])
 def test_factorize_eigensolver_paths(ctx, m, n, maxdim):
     """Gram sizes across every eigensolver path, with a graded spectrum: the
     batched Jacobi solver for small Gram matrices, every plan of the one-launch
-    tridiagonal reduction (one workgroup to 64; 4, 4, 6, 8 rows per wave to 128,
-    256, 384, 512 on up to 16 workgroups; the cooperative grid to 1024) on both
+    tridiagonal reduction (one workgroup to 64; 4, 4, 6, 7, 8 rows per wave to 128,
+    256, 384, 448, 512 on up to 16 workgroups; the cooperative grid to 1024) on both
     sides of each boundary, and rocSOLVER dsyevd above 1024."""
     import networksolvers_amd as ns
     rng = np.random.default_rng(m + n)
