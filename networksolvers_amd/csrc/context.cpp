@@ -109,7 +109,76 @@ void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t) {
 }
 
 void* scratch(nsmi_ctx* ctx, size_t bytes) { return ctx->pool->alloc(bytes); }
-void scratch_release(nsmi_ctx* ctx, void* p) { ctx->pool->release(p); }
+void scratch_release(nsmi_ctx* ctx, void* p) {
+  if (ctx->defer_release)
+    ctx->deferred.push_back(p);
+  else
+    ctx->pool->release(p);
+}
+
+AuxSection::AuxSection(nsmi_ctx* c, int want) : ctx(c), k(0) {
+  while (int(ctx->aux_stream.size()) < want) {
+    hipStream_t s = nullptr;
+    rocblas_handle h = nullptr;
+    int* info = nullptr;
+    bool ok = hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && rocblas_create_handle(&h) == rocblas_status_success;
+    ok = ok && rocblas_set_stream(h, s) == rocblas_status_success;
+    ok = ok && hipMalloc(&info, sizeof(int) * 4) == hipSuccess;
+    if (!ok) {
+      if (info) (void)hipFree(info);
+      if (h) rocblas_destroy_handle(h);
+      if (s) (void)hipStreamDestroy(s);
+      fail(NSMI_ERR_HIP, "auxiliary stream set-up failed");
+    }
+    ctx->aux_stream.push_back(s);
+    ctx->aux_blas.push_back(h);
+    ctx->aux_info.push_back(info);
+  }
+  while (ctx->aux_event.size() < ctx->aux_stream.size() + 1) {
+    hipEvent_t e = nullptr;
+    NSMI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->aux_event.push_back(e);
+  }
+  k = want;
+  ctx->defer_release = true;
+}
+
+AuxSection::~AuxSection() {
+  for (int j = 0; j < k; ++j) (void)hipStreamSynchronize(ctx->aux_stream[j]);
+  ctx->defer_release = false;
+  for (void* p : ctx->deferred) ctx->pool->release(p);
+  ctx->deferred.clear();
+}
+
+void AuxSection::fork() {
+  NSMI_HIP(hipEventRecord(ctx->aux_event[0], ctx->stream));
+  for (int j = 0; j < k; ++j) NSMI_HIP(hipStreamWaitEvent(ctx->aux_stream[j], ctx->aux_event[0], 0));
+}
+
+void AuxSection::join() {
+  for (int j = 0; j < k; ++j) {
+    NSMI_HIP(hipEventRecord(ctx->aux_event[1 + j], ctx->aux_stream[j]));
+    NSMI_HIP(hipStreamWaitEvent(ctx->stream, ctx->aux_event[1 + j], 0));
+  }
+}
+
+void AuxSection::sync() {
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  for (int j = 0; j < k; ++j) NSMI_HIP(hipStreamSynchronize(ctx->aux_stream[j]));
+}
+
+OnAux::OnAux(nsmi_ctx* c, int j) : ctx(c), s(c->stream), b(c->blas), info(c->d_info) {
+  ctx->stream = ctx->aux_stream[j];
+  ctx->blas = ctx->aux_blas[j];
+  ctx->d_info = ctx->aux_info[j];
+}
+
+OnAux::~OnAux() {
+  ctx->stream = s;
+  ctx->blas = b;
+  ctx->d_info = info;
+}
 
 static hipEvent_t take_event(nsmi_ctx* ctx) {
   if (!ctx->spare_events.empty()) {
@@ -228,6 +297,13 @@ int nsmi_ctx_destroy(nsmi_ctx* ctx) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  for (size_t j = 0; j < ctx->aux_stream.size(); ++j) {
+    (void)hipStreamSynchronize(ctx->aux_stream[j]);
+    if (j < ctx->aux_blas.size()) rocblas_destroy_handle(ctx->aux_blas[j]);
+    if (j < ctx->aux_info.size()) (void)hipFree(ctx->aux_info[j]);
+    (void)hipStreamDestroy(ctx->aux_stream[j]);
+  }
+  for (auto e : ctx->aux_event) (void)hipEventDestroy(e);
   if (ctx->blas) rocblas_destroy_handle(ctx->blas);
   delete ctx->pool;
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);

@@ -574,9 +574,11 @@ int sytrd_coop_groups(int n) {
 
 // A: n x n symmetric on the device (row-major == column-major), overwritten with
 // LAPACK's lower-triangle tridiagonal layout; d (n), e and tau (n - 1) on the
-// device.  Returns false, with nothing launched, when n is outside the supported
-// range or the device cannot co-schedule the grid (the caller then uses the
-// library path).
+// device.  sytrd_launch enqueues the reduction on ctx->stream and the copy of its
+// two failure flags into run.host_flags; it returns false, with nothing launched,
+// when n is outside the supported range or the device cannot co-schedule the
+// grid (the caller then uses the library path).  sytrd_check reads the flags
+// once the stream has passed the copy.
 //
 // Grids of up to kPlainLaunchMax workgroups go out as ordinary launches: a
 // cooperative launch waits for the whole device to drain, which would serialise
@@ -584,7 +586,7 @@ int sytrd_coop_groups(int n) {
 // of 256) are co-resident in practice even beside other concurrent reductions;
 // the bounded barrier spin turns the pathological case into an error, not a
 // hang.  Larger grids use the cooperative launch and its co-residency check.
-bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau) {
+bool sytrd_launch(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau, SytrdRun& run) {
   constexpr int kPlainLaunchMax = 16;
   const int G = sytrd_coop_groups(n);
   if (G == 0) return false;
@@ -597,13 +599,13 @@ bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* t
     NSMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.fn, kTriThreads, 0));
     if (!coop || per_cu < 1 || G > ncu * per_cu) return false;
   }
-  Scratch xch(ctx, sizeof(double) * 4 * size_t(n));
-  Scratch flags(ctx, 256);
-  unsigned* bar = flags.as<unsigned>();
+  run.xch.reset(new Scratch(ctx, sizeof(double) * 4 * size_t(n)));
+  run.flags.reset(new Scratch(ctx, 256));
+  unsigned* bar = run.flags->as<unsigned>();
   unsigned* failp = bar + 32;
-  NSMI_HIP(hipMemsetAsync(flags.p, 0, 256, ctx->stream));
+  NSMI_HIP(hipMemsetAsync(run.flags->p, 0, 256, ctx->stream));
   int nn = n;
-  double* xp = xch.as<double>();
+  double* xp = run.xch->as<double>();
   void* args[] = {&A, &nn, &d, &e, &tau, &xp, &bar, &failp};
   {
     ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 4.0 / 3.0 * double(n) * n * n, 0.0);
@@ -618,13 +620,16 @@ bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* t
       NSMI_HIP(hipLaunchKernel(p.fn, dim3(unsigned(G)), dim3(kTriThreads), args, 0, ctx->stream));
     }
   }
-  unsigned f[2] = {0u, 0u};
-  NSMI_HIP(hipMemcpyAsync(f, failp, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
-  NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  if (f[0]) fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
-  // the tridiagonal solver is never handed NaN or Inf
-  if (f[1]) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite entries in the matrix to be factorized");
+  run.host_flags[0] = run.host_flags[1] = 0u;
+  NSMI_HIP(hipMemcpyAsync(run.host_flags, failp, sizeof(run.host_flags), hipMemcpyDeviceToHost, ctx->stream));
   return true;
+}
+
+void sytrd_check(const SytrdRun& run) {
+  if (run.host_flags[0])
+    fail(NSMI_ERR_SOLVER, "sytrd: a workgroup of the tridiagonal reduction timed out at a grid barrier");
+  // the tridiagonal solver is never handed NaN or Inf
+  if (run.host_flags[1]) fail(NSMI_ERR_NOT_FINITE, "factorize: non-finite entries in the matrix to be factorized");
 }
 
 // Eigen-decomposition of every listed matrix (n <= kJacobiMax) in one launch.

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <memory>
 #include <numeric>
 
@@ -23,6 +24,9 @@ using namespace nsmi;
 
 namespace nsmi {
 namespace {
+
+// auxiliary streams for the sector solves of one QN bond
+constexpr size_t kAuxStreams = 4;
 
 // restatement of NDTensors truncate! (relative cutoff); returns kept count
 int64_t truncate_spectrum(std::vector<double>& P, int64_t mindim, int64_t maxdim, double cutoff, double* truncerr,
@@ -221,7 +225,10 @@ struct FactCore {
   bool jacobi = false;  // rho awaits the batched Jacobi solve (eigenvalues into wdev)
   std::vector<double> P;
   std::vector<int> order;
-  std::unique_ptr<Scratch> rho, z, tau, wdev;
+  std::unique_ptr<Scratch> rho, z, tau, wdev, w, e;
+  SytrdRun sr;             // the tridiagonal reduction's buffers and host flags
+  std::vector<double> wh;  // host copy of the eigenvalues (filled asynchronously)
+  int info_h = 0;          // host copy of the solver status
   FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
                                                                          route(r) {
     iso = rt ? nc : m;
@@ -249,91 +256,134 @@ void fact_spectrum(FactCore& f, std::vector<double>& wh) {
   for (int64_t i = 0; i < iso; ++i) f.P[i] = wh[f.order[i]];
 }
 
-void fact_gram(nsmi_ctx* ctx, FactCore& f) {
+// Phases of one factorisation, so that several independent ones (the sectors of a
+// QN bond) can run on several streams: the Gram product, then for a small rho the
+// batched Jacobi solve; otherwise the one-launch reduction and, once its flags are
+// read, rocSOLVER's divide and conquer on T (the back-transformation is deferred
+// to fact_emit, where only the kept vectors are transformed), or the library's
+// dsyevd when the reduction cannot run.  Every phase only enqueues; fact_finish
+// reads the copied status and eigenvalues after the caller's synchronisation.
+void fact_gram_gemm(nsmi_ctx* ctx, FactCore& f) {
   // The "svd" route diagonalises the Gram matrix as well: rocSOLVER's dgesvd is
   // ~80x slower than dsyevd at n = 2048 on gfx950 (3.9 s vs 47 ms) and 8x slower
   // than the batched Jacobi solver at n = 32 (4.2 vs 0.55 ms per factorisation).
   const int64_t iso = f.iso, oth = f.oth, m = f.m, nc = f.nc;
+  f.rho.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
   {
-    f.rho.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
-    {
-      ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
-      if (!f.rows_trailing)
-        gemm_f64(ctx, gdesc(m, m, nc, false, true), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
-      else
-        gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
-    }
-    if (iso <= kJacobiMax) {
-      // small Gram matrices go to the batched Jacobi solver (fact_jacobi)
-      f.jacobi = true;
-      f.wdev.reset(new Scratch(ctx, sizeof(double) * size_t(iso)));
-      return;
-    }
-    Scratch w(ctx, sizeof(double) * size_t(iso));
-    Scratch e(ctx, sizeof(double) * size_t(iso));
-    // Tridiagonal route: the cooperative reduction, then rocSOLVER's divide and
-    // conquer on T; the back-transformation is deferred to fact_emit, where only
-    // the kept vectors are transformed.  Otherwise the library's dsyevd.
-    f.tau.reset(new Scratch(ctx, sizeof(double) * size_t(std::max<int64_t>(iso, 2))));
-    f.tri = sytrd_coop(ctx, f.rho->as<double>(), int(iso), w.as<double>(), e.as<double>(), f.tau->as<double>());
-    if (f.tri) {
-      f.z.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
-      {
-        ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 4.0 * double(iso) * iso * iso, 0.0);
-        check_rocsolver(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, rocblas_int(iso), w.as<double>(),
-                                         e.as<double>(), f.z->as<double>(), rocblas_int(iso), ctx->d_info),
-                        "rocsolver_dstedc");
-      }
-    } else {
-      f.tau.reset();
-      require_finite(ctx, f.rho->as<double>(), iso * iso, "factorize");
-      {
-        ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
-        check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
-                                         f.rho->as<double>(), rocblas_int(iso), w.as<double>(), e.as<double>(),
-                                         ctx->d_info),
-                        "rocsolver_dsyevd");
-      }
-    }
-    // the solver's status and the eigenvalues in one round trip
-    std::vector<double> wh(iso);
-    int info = 0;
-    NSMI_HIP(hipMemcpyAsync(&info, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    NSMI_HIP(hipMemcpyAsync(wh.data(), w.p, sizeof(double) * iso, hipMemcpyDeviceToHost, ctx->stream));
-    NSMI_HIP(hipStreamSynchronize(ctx->stream));
-    if (info != 0)
-      fail(NSMI_ERR_SOLVER, std::string(f.tri ? "rocsolver_dstedc" : "rocsolver_dsyevd") + " returned info = " +
-                                std::to_string(info));
-    fact_spectrum(f, wh);
+    ProfScope ps(ctx, NSMI_K_FACT_GEMM, 2.0 * double(iso) * iso * oth, 8.0 * (double(m) * nc + double(iso) * iso));
+    if (!f.rows_trailing)
+      gemm_f64(ctx, gdesc(m, m, nc, false, true), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
+    else
+      gemm_f64(ctx, gdesc(nc, nc, m, true, false), 1.0, f.M, f.M, 0.0, f.rho->as<double>(), kRoleFact);
   }
+  if (iso <= kJacobiMax) {
+    f.jacobi = true;
+    f.wdev.reset(new Scratch(ctx, sizeof(double) * size_t(iso)));
+    return;
+  }
+  f.w.reset(new Scratch(ctx, sizeof(double) * size_t(iso)));
+  f.e.reset(new Scratch(ctx, sizeof(double) * size_t(iso)));
+  f.tau.reset(new Scratch(ctx, sizeof(double) * size_t(std::max<int64_t>(iso, 2))));
+}
+
+// the reduction; false when it cannot run here (the caller takes fact_syevd)
+bool fact_tri_launch(nsmi_ctx* ctx, FactCore& f) {
+  f.tri = sytrd_launch(ctx, f.rho->as<double>(), int(f.iso), f.w->as<double>(), f.e->as<double>(),
+                       f.tau->as<double>(), f.sr);
+  return f.tri;
+}
+
+void fact_readback(nsmi_ctx* ctx, FactCore& f) {
+  f.wh.assign(size_t(f.iso), 0.0);
+  f.info_h = 0;
+  NSMI_HIP(hipMemcpyAsync(&f.info_h, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipMemcpyAsync(f.wh.data(), f.w->p, sizeof(double) * f.iso, hipMemcpyDeviceToHost, ctx->stream));
+}
+
+// after the reduction's flags are on the host
+void fact_tri_solve(nsmi_ctx* ctx, FactCore& f) {
+  sytrd_check(f.sr);
+  const int64_t iso = f.iso;
+  f.z.reset(new Scratch(ctx, sizeof(double) * size_t(iso) * iso));
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 4.0 * double(iso) * iso * iso, 0.0);
+    check_rocsolver(rocsolver_dstedc(ctx->blas, rocblas_evect_tridiagonal, rocblas_int(iso), f.w->as<double>(),
+                                     f.e->as<double>(), f.z->as<double>(), rocblas_int(iso), ctx->d_info),
+                    "rocsolver_dstedc");
+  }
+  fact_readback(ctx, f);
+}
+
+// synchronises (the finiteness check reads a device flag)
+void fact_syevd(nsmi_ctx* ctx, FactCore& f) {
+  const int64_t iso = f.iso;
+  f.tau.reset();
+  require_finite(ctx, f.rho->as<double>(), iso * iso, "factorize");
+  {
+    ProfScope ps(ctx, NSMI_K_FACT_SOLVE, 9.0 * double(iso) * iso * iso, 0.0);
+    check_rocsolver(rocsolver_dsyevd(ctx->blas, rocblas_evect_original, rocblas_fill_upper, rocblas_int(iso),
+                                     f.rho->as<double>(), rocblas_int(iso), f.w->as<double>(), f.e->as<double>(),
+                                     ctx->d_info),
+                    "rocsolver_dsyevd");
+  }
+  fact_readback(ctx, f);
+}
+
+// after the stream has passed fact_readback's copies
+void fact_finish(FactCore& f) {
+  if (f.info_h != 0)
+    fail(NSMI_ERR_SOLVER, std::string(f.tri ? "rocsolver_dstedc" : "rocsolver_dsyevd") + " returned info = " +
+                              std::to_string(f.info_h));
+  fact_spectrum(f, f.wh);
+}
+
+// one factorisation on the main stream
+void fact_gram(nsmi_ctx* ctx, FactCore& f) {
+  fact_gram_gemm(ctx, f);
+  if (f.jacobi) return;
+  if (fact_tri_launch(ctx, f)) {
+    NSMI_HIP(hipStreamSynchronize(ctx->stream));
+    fact_tri_solve(ctx, f);
+  } else {
+    fact_syevd(ctx, f);
+  }
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  fact_finish(f);
 }
 
 
-// The deferred small Gram matrices of several cores in one batched launch.
-void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
+// The deferred small Gram matrices of several cores in one batched launch:
+// enqueue (eigenvalues and the convergence flag copied to the host), then finish
+// after the caller's synchronisation.
+void fact_jacobi_launch(nsmi_ctx* ctx, const std::vector<FactCore*>& fs, unsigned* unconverged) {
   std::vector<JacobiJob> jobs;
   for (FactCore* f : fs)
     if (f->jacobi) jobs.push_back(JacobiJob{f->rho->as<double>(), f->wdev->as<double>(), int(f->iso)});
+  *unconverged = 0u;
   if (jobs.empty()) return;
-  unsigned unconverged = 0;
-  jacobi_eig_batched(ctx, jobs, &unconverged);
-  std::vector<std::vector<double>> wh;
-  wh.reserve(jobs.size());
+  jacobi_eig_batched(ctx, jobs, unconverged);
   for (FactCore* f : fs)
     if (f->jacobi) {
-      wh.emplace_back(size_t(f->iso));
-      NSMI_HIP(hipMemcpyAsync(wh.back().data(), f->wdev->p, sizeof(double) * f->iso, hipMemcpyDeviceToHost,
+      f->wh.assign(size_t(f->iso), 0.0);
+      NSMI_HIP(hipMemcpyAsync(f->wh.data(), f->wdev->p, sizeof(double) * f->iso, hipMemcpyDeviceToHost,
                               ctx->stream));
     }
-  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void fact_jacobi_finish(const std::vector<FactCore*>& fs, unsigned unconverged) {
   if (unconverged) fail(NSMI_ERR_SOLVER, "jacobi: a Gram matrix did not converge within the sweep limit");
-  size_t j = 0;
   for (FactCore* f : fs)
     if (f->jacobi) {
-      fact_spectrum(*f, wh[j++]);
+      fact_spectrum(*f, f->wh);
       f->jacobi = false;
-      f->wdev.reset();
     }
+}
+
+void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
+  unsigned unconverged = 0;
+  fact_jacobi_launch(ctx, fs, &unconverged);
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  fact_jacobi_finish(fs, unconverged);
 }
 
 void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
@@ -614,15 +664,53 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
   require(route == NSMI_ROUTE_SVD || route == NSMI_ROUTE_EIGEN, "factorize_sectors: unknown route");
   std::vector<std::unique_ptr<FactCore>> fc;
   int64_t full = 0;
-  std::vector<FactCore*> fcp;
+  std::vector<FactCore*> fcp, big, lib;
   for (int q = 0; q < n; ++q) {
     fc.emplace_back(new FactCore(static_cast<const double*>(Mall->data) + offsets[q], rows[q], cols[q],
                                  rows_trailing != 0, route));
-    fact_gram(ctx, *fc.back());
+    fact_gram_gemm(ctx, *fc.back());
     fcp.push_back(fc.back().get());
     full += fc.back()->full;
+    if (!fc.back()->jacobi) big.push_back(fc.back().get());
   }
-  fact_jacobi(ctx, fcp);  // every small sector's eigensolve in one launch
+  // The sectors are independent: every small one goes into one batched Jacobi
+  // launch on the main stream, and the reductions and divide-and-conquer solves of
+  // the larger ones are spread over auxiliary streams (each solve is a chain of
+  // latency-bound launches, so several overlap on the device); a sector whose
+  // reduction cannot run takes dsyevd on the main stream.
+  const int naux = int(std::min<size_t>(big.size(), kAuxStreams));
+  std::unique_ptr<AuxSection> sec;
+  if (naux > 1) {
+    sec.reset(new AuxSection(ctx, naux));
+    sec->fork();
+  }
+  auto on_stream = [&](size_t t, const std::function<void()>& fn) {
+    if (sec) {
+      OnAux a(ctx, int(t % size_t(naux)));
+      fn();
+    } else {
+      fn();
+    }
+  };
+  auto sync_all = [&]() {
+    if (sec)
+      sec->sync();
+    else
+      NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  };
+  for (size_t t = 0; t < big.size(); ++t)
+    on_stream(t, [&]() {
+      if (!fact_tri_launch(ctx, *big[t])) lib.push_back(big[t]);
+    });
+  unsigned unconverged = 0;
+  fact_jacobi_launch(ctx, fcp, &unconverged);
+  sync_all();
+  for (size_t t = 0; t < big.size(); ++t)
+    if (big[t]->tri) on_stream(t, [&]() { fact_tri_solve(ctx, *big[t]); });
+  for (FactCore* f : lib) fact_syevd(ctx, *f);
+  sync_all();
+  fact_jacobi_finish(fcp, unconverged);
+  for (FactCore* f : big) fact_finish(*f);
   auto key = [&](double v) { return route == NSMI_ROUTE_EIGEN ? std::fabs(v) : v; };
   struct Entry {
     double key, value;
@@ -665,13 +753,18 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
   nsmi_tensor* Ct = tensor_new(ctx, NSMI_F64, 1, cd);
   made.push_back(Ct);
   int64_t uo = 0, co = 0;
+  if (sec) sec->fork();  // the outputs were allocated on the main stream
+  size_t t = 0;
   for (int q = 0; q < n; ++q) {
     const int64_t k = kept[q];
     if (k == 0) continue;
-    fact_emit(ctx, *fc[q], k, static_cast<double*>(Ut->data) + uo, static_cast<double*>(Ct->data) + co);
+    double* U = static_cast<double*>(Ut->data) + uo;
+    double* C = static_cast<double*>(Ct->data) + co;
+    on_stream(t++, [&]() { fact_emit(ctx, *fc[q], k, U, C); });
     uo += k * (rows_trailing ? cols[q] : rows[q]);
     co += k * (rows_trailing ? rows[q] : cols[q]);
   }
+  if (sec) sec->join();
   if (info) {
     info->kept = sum;
     info->full = full;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -86,6 +87,16 @@ struct nsmi_ctx {
   double* d_scal = nullptr;      // 64 result slots
   double* h_scal = nullptr;      // pinned mirror of d_scal
   int* d_info = nullptr;
+  // auxiliary streams (default priority), each with its own rocBLAS handle and
+  // solver status word, for independent sector solves; created on first use
+  std::vector<hipStream_t> aux_stream;
+  std::vector<rocblas_handle> aux_blas;
+  std::vector<int*> aux_info;
+  std::vector<hipEvent_t> aux_event;  // [0]: fork from the main stream, [1 + j]: join of aux j
+  // while set, released pool blocks are held back (work on several streams may
+  // still use them) and returned once every stream has drained
+  bool defer_release = false;
+  std::vector<void*> deferred;
   // profiling
   bool prof_on = false;
   unsigned prof_mask = ~0u;  // classes recorded while prof_on
@@ -115,6 +126,30 @@ nsmi_tensor* tensor_new(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims)
 void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t);
 void* scratch(nsmi_ctx* ctx, size_t bytes);        // pool block, caller releases
 void scratch_release(nsmi_ctx* ctx, void* p);
+
+// Work on the auxiliary streams: AuxSection creates them (k at most), holds pool
+// releases back for its lifetime and, on exit, waits for every auxiliary stream
+// before the held blocks go back to the pool; fork() orders the auxiliary streams
+// after everything enqueued so far on the main stream, join() the main stream
+// after everything enqueued on them.  OnAux points ctx->stream, ctx->blas and
+// ctx->d_info at auxiliary stream j for its scope.
+struct AuxSection {
+  nsmi_ctx* ctx;
+  int k;
+  AuxSection(nsmi_ctx* c, int want);
+  ~AuxSection();
+  void fork();
+  void join();
+  void sync();  // host wait for the main and every auxiliary stream
+};
+struct OnAux {
+  nsmi_ctx* ctx;
+  hipStream_t s;
+  rocblas_handle b;
+  int* info;
+  OnAux(nsmi_ctx* c, int j);
+  ~OnAux();
+};
 
 // RAII holder for pool scratch
 struct Scratch {
@@ -243,7 +278,14 @@ void small_sym_eigh(int n, const double* A, double* w, double* V);  // host Jaco
 // [2, kSytrdCoopMax] or the device cannot co-schedule the grid.
 constexpr int kSytrdCoopMax = 1024;
 int sytrd_coop_groups(int n);
-bool sytrd_coop(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau);
+// one asynchronous tridiagonal reduction: its exchange and flag buffers, and the
+// host copy of the two failure flags (barrier time-out, non-finite d or e)
+struct SytrdRun {
+  std::unique_ptr<Scratch> xch, flags;
+  unsigned host_flags[2] = {0u, 0u};
+};
+bool sytrd_launch(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau, SytrdRun& run);
+void sytrd_check(const SytrdRun& run);
 // Batched Jacobi eigensolver for small symmetric matrices (one workgroup each, LDS
 // resident): eigenvectors overwrite a (column-major, dsyevd layout), eigenvalues
 // (unsorted) go to w.  Enqueues only: *unconverged (host memory) receives the
