@@ -27,7 +27,8 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16": 2500.0}
 HBM_PEAK_GBS = 8000.0
 
 
-PROFILE_EVERY = 8    # H_eff GEMM launches per HIP-event sample in the timed sweeps
+PROFILE_EVERY = 7    # H_eff GEMM launches per HIP-event sample in the timed sweeps (prime, so the
+                     # sample walks through the positions of the launches within a region)
 
 
 def parse():
