@@ -132,3 +132,39 @@ def test_expm_rejects_non_finite_input(ctx, tau):
         ns.krylov_expm(Hp, ctx.upload(x0), tau, krylovdim=4)
     assert err.value.code == NOT_FINITE
     _lanczos_still_works(ctx)
+
+
+@pytest.mark.parametrize("bad_sector", [0, 1, 2])
+def test_sector_factorize_rejects_non_finite(ctx, bad_sector):
+    """The sector factorisation (independent sectors on several streams: two on the
+    reduction path, one on the batched Jacobi path) reports a NaN in any sector as
+    NSMI_ERR_NOT_FINITE, and the next call on the same context is exact."""
+    from networksolvers_amd import qn
+    from networksolvers_amd._lib import NsmiError
+    from oracle import qn as oqn
+    shapes = [(80, 90), (60, 70), (30, 35)]
+    rng = np.random.default_rng(9)
+    mats = [_graded(m, n, 20 + i) for i, (m, n) in enumerate(shapes)]
+    offs, flat = [], []
+    for M in mats:
+        offs.append(sum(x.size for x in flat))
+        flat.append(M.reshape(-1))
+    buf = np.concatenate(flat)
+    bad = buf.copy()
+    bad[offs[bad_sector] + 7] = np.nan
+    rows, cols = [m for m, _ in shapes], [n for _, n in shapes]
+    with pytest.raises(NsmiError, match="non-finite") as err:
+        qn._factorize_sectors(ctx, ctx.upload(bad), offs, rows, cols, False, 1e-10, None, 1, "eigen")
+    assert err.value.code == NOT_FINITE
+    U, C, kept, info = qn._factorize_sectors(ctx, ctx.upload(buf), offs, rows, cols, False, 1e-10, None, 1, "eigen")
+    res, okept, _ = oqn.factorize_sectors(mats, False, cutoff=1e-10, which_decomp="eigen")
+    assert kept == okept
+    Uh, Ch = U.numpy(), C.numpy()
+    uo = co = 0
+    for M, k, (m, n) in zip(mats, kept, shapes):
+        Uq = Uh[uo:uo + m * k].reshape(m, k)
+        Cq = Ch[co:co + k * n].reshape(k, n)
+        np.testing.assert_allclose(Uq.T @ Uq, np.eye(k), atol=1e-12)
+        np.testing.assert_allclose(Uq @ Cq, M, atol=1e-9)
+        uo += m * k
+        co += k * n
