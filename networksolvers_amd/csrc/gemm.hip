@@ -65,7 +65,11 @@ __device__ inline int xcd_remap(int wg, int nwg) {
 
 template <typename T, typename TBm, typename TCm, int BM, int BN, bool TA, bool TB, int ROLE>
 __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
-  constexpr int BK = 16, PAD = 4;
+  // the small tiles run twice as deep a K slice per LDS stage: the products they
+  // serve have few tiles and long K, so the per-stage barrier and load latency is
+  // what they pay (measured on the FP32 H_eff GEMM2 at 852 x 213 x 1065: 38.8 us
+  // at depth 16, 27.7 us at 32, 30.8 us at 64)
+  constexpr int BK = (BM == 32 && BN == 32) ? 32 : 16, PAD = 4;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int FM = WTM / 16, FN = WTN / 16;
   constexpr int EA = BM * BK / 256, EB = BN * BK / 256;
