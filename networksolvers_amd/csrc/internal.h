@@ -240,6 +240,11 @@ void vec_axpy2_norm2_dev(nsmi_ctx* ctx, double* w, const double* v1, double c1, 
                          double c2, int64_t n, int slot);
 // out = sum_j c[j] * V[j]
 void vec_lincomb(nsmi_ctx* ctx, double* out, const double* const* V, const double* c, int k, int64_t n);
+// y += a x with the coefficient a = f * slot, or f * (slot / div) when div != 0, read
+// on the device (no host round trip); complex: a = f * (slot, slot + 1), or
+// (f * slot / div, 0) when div != 0
+void vec_axpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div);
+void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div);
 // read result slots [0, count) to host (synchronises)
 void read_slots(nsmi_ctx* ctx, int count, double* out);
 void vec_to_f32(nsmi_ctx* ctx, float* y, const double* x, int64_t n);

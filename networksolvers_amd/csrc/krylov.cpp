@@ -60,7 +60,26 @@ struct KOps {
   }
   size_t bytes() const { return size_t(n) * (is_c ? 16 : 8); }
   size_t doubles() const { return size_t(n) * (is_c ? 2 : 1); }
+  // device-resident variants: the result stays in a slot until read() collects
+  // several at once, and axpy_dc takes its coefficient from a slot
+  void dot_dev(const double* a, const double* b, int slot) const {  // sum conj(a) b -> slot (and slot + 1)
+    if (is_c)
+      vec_zdot_dev(ctx, a, b, n, slot);
+    else
+      vec_dot_dev(ctx, a, b, n, slot);
+  }
+  void norm2_dev(const double* a, int slot) const { vec_dot_dev(ctx, a, a, is_c ? 2 * n : n, slot); }
+  // y += f * c x with c the (complex) slot value, or y += f * (Re c / div) x when div != 0
+  void axpy_dc(double* y, const double* x, int slot, double f, double div = 0.0) const {
+    if (is_c)
+      vec_zaxpy_dc(ctx, y, x, n, slot, f, div);
+    else
+      vec_axpy_dc(ctx, y, x, n, slot, f, div);
+  }
 };
+
+// result slots of the recurrences: a dot product (two slots when complex) and two squared norms
+constexpr int kSlotDot = 0, kSlotN2a = 2, kSlotN2b = 3;
 
 const double kEps = 2.220446049250313e-16;
 
@@ -75,30 +94,37 @@ void require_finite_coefficients(double alpha, double beta) {
 // with norm beta (r orthogonal to V up to rounding), appends v_k = r / beta, applies
 // H, and returns the new (alpha_k, beta_k) with r the new residual.  Three-term
 // recurrence, then modified Gram-Schmidt against the whole basis while the norm
-// drops below eta of its previous value (KrylovKit ModifiedGramSchmidtIR).
+// drops below eta of its previous value (KrylovKit ModifiedGramSchmidtIR).  The
+// Gram-Schmidt coefficients stay on the device (each axpy reads the dot product
+// before it from a result slot), so the host reads back once per pass: the last
+// coefficient and the new norm.
 void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r, double beta, double eta,
                     double* alpha_out, double* beta_out) {
   const size_t nd = ops.doubles();
   double* vk = V + nd * k;
   double* vprev = V + nd * (k - 1);
+  double sl[4];
   ops.scaled_copy(vk, r, 1.0 / beta);
   heff_apply_dev(ops.ctx, op, vk, r);
   ops.axpy(r, vprev, cplx(-beta, 0.0));
-  cplx c = ops.dot(vk, r);
-  double a = c.real();
-  ops.axpy(r, vk, -c);
-  double b = ops.norm(r);
+  ops.dot_dev(vk, r, kSlotDot);
+  ops.axpy_dc(r, vk, kSlotDot, -1.0);
+  ops.norm2_dev(r, kSlotN2a);
+  read_slots(ops.ctx, 3, sl);
+  double a = sl[0];
+  double b = std::sqrt(std::max(sl[kSlotN2a], 0.0));
   double nold = std::sqrt(b * b + a * a + beta * beta);
   while (kEps < b && b < eta * nold) {
     nold = b;
-    cplx cl;
     for (int j = 0; j <= k; ++j) {
       const double* q = V + nd * j;
-      cl = ops.dot(q, r);
-      ops.axpy(r, q, -cl);
+      ops.dot_dev(q, r, kSlotDot);
+      ops.axpy_dc(r, q, kSlotDot, -1.0);
     }
-    a += cl.real();
-    b = ops.norm(r);
+    ops.norm2_dev(r, kSlotN2a);
+    read_slots(ops.ctx, 3, sl);
+    a += sl[0];  // the last coefficient (against v_k)
+    b = std::sqrt(std::max(sl[kSlotN2a], 0.0));
   }
   require_finite_coefficients(a, b);
   *alpha_out = a;
@@ -108,20 +134,26 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
 // first Lanczos vector from x0: V[0] = x0 / |x0|, r = H V[0] - alpha V[0], reorthogonalised
 void lanczos_start(const KOps& ops, nsmi_heff* op, const double* x0, double beta0, double* V, double* r,
                    double eta, double* alpha_out, double* beta_out) {
+  double sl[4];
   heff_apply_dev(ops.ctx, op, x0, r);  // H x0
-  cplx c = ops.dot(x0, r);
-  double alpha = c.real() / (beta0 * beta0);
+  ops.dot_dev(x0, r, kSlotDot);        // alpha = Re <x0, H x0> / beta0^2 (read below)
   ops.scaled_copy(V, x0, 1.0 / beta0);
   ops.scale(r, 1.0 / beta0);
-  double beta_old = ops.norm(r);
-  ops.axpy(r, V, cplx(-alpha, 0.0));
-  double beta = ops.norm(r);
+  ops.norm2_dev(r, kSlotN2a);
+  ops.axpy_dc(r, V, kSlotDot, -1.0, beta0 * beta0);
+  ops.norm2_dev(r, kSlotN2b);
+  read_slots(ops.ctx, 4, sl);
+  double alpha = sl[0] / (beta0 * beta0);
+  double beta_old = std::sqrt(std::max(sl[kSlotN2a], 0.0));
+  double beta = std::sqrt(std::max(sl[kSlotN2b], 0.0));
   while (kEps < beta && beta < eta * beta_old) {
     beta_old = beta;
-    cplx d = ops.dot(V, r);
-    alpha += d.real();
-    ops.axpy(r, V, -d);
-    beta = ops.norm(r);
+    ops.dot_dev(V, r, kSlotDot);
+    ops.axpy_dc(r, V, kSlotDot, -1.0);
+    ops.norm2_dev(r, kSlotN2b);
+    read_slots(ops.ctx, 4, sl);
+    alpha += sl[0];
+    beta = std::sqrt(std::max(sl[kSlotN2b], 0.0));
   }
   require_finite_coefficients(alpha, beta);
   *alpha_out = alpha;

@@ -28,6 +28,15 @@ __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, int
     y[i] += a * x[i];
 }
 
+// y += a x with a = f c[0], or f (c[0] / div) when div != 0, read from device memory
+// (a coefficient that a previous reduction left in a result slot)
+__global__ void k_axpy_dc(double* __restrict__ y, const double* __restrict__ x, int64_t n, const double* c, double f,
+                          double div) {
+  const double a = div != 0.0 ? f * (c[0] / div) : f * c[0];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] += a * x[i];
+}
+
 __global__ void k_scaled_copy(double* __restrict__ y, const double* __restrict__ x, int64_t n, double a) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
     y[i] = a * x[i];
@@ -155,6 +164,18 @@ __global__ void k_zaxpy(double* __restrict__ y, const double* __restrict__ x, in
   }
 }
 
+// complex y += a x with a = (f c[0], f c[1]), or (f (c[0] / div), 0) when div != 0
+__global__ void k_zaxpy_dc(double* __restrict__ y, const double* __restrict__ x, int64_t n, const double* c, double f,
+                           double div) {
+  const double ar = div != 0.0 ? f * (c[0] / div) : f * c[0];
+  const double ai = div != 0.0 ? 0.0 : f * c[1];
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    y[2 * i] += ar * xr - ai * xi;
+    y[2 * i + 1] += ar * xi + ai * xr;
+  }
+}
+
 // partials[b] = sum conj(x) y  (re), partials[kPartials + b] (im)
 __global__ void k_zdot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
                                double* __restrict__ partials) {
@@ -245,6 +266,24 @@ void vec_axpy(nsmi_ctx* ctx, double* y, const double* x, int64_t n, double a) {
   if (n <= 0) return;
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 24.0 * n);
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, a);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_axpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div) {
+  require(slot >= 0 && slot < 64, "slot out of range");
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 24.0 * n);
+  hipLaunchKernelGGL(k_axpy_dc, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, ctx->d_scal + slot, f,
+                     div);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div) {
+  require(slot >= 0 && slot + 1 < 64, "slot out of range");
+  if (n <= 0) return;
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 48.0 * n);
+  hipLaunchKernelGGL(k_zaxpy_dc, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, ctx->d_scal + slot, f,
+                     div);
   NSMI_LAUNCH_CHECK();
 }
 
