@@ -45,28 +45,42 @@ namespace {
 
 constexpr int kMixThreads = 256;
 
+// Y[o, p, i] = sum_q Mx[p, q] X[o, q, i]: one thread per (o, i) holds the Q inputs in registers;
+// blockIdx.z selects a chunk of PC outputs, so small operators still fill the device.  The
+// coefficient addresses are wave-uniform (scalar loads), and four outputs accumulate at a time
+// as independent chains.  Every output sums q in ascending order, whatever the chunking.
 template <typename TI, typename TO, int QMAX>
 __global__ __launch_bounds__(kMixThreads) void k_mix(const TI* __restrict__ X, TO* __restrict__ Y,
                                                      const double* __restrict__ Mx, int64_t O, int P, int Q,
-                                                     int64_t I) {
-  __shared__ double sM[64 * 64];
-  for (int t = threadIdx.x; t < P * Q; t += blockDim.x) sM[t] = Mx[t];
-  __syncthreads();
+                                                     int64_t I, int PC) {
   const int64_t i = int64_t(blockIdx.x) * kMixThreads + threadIdx.x;
   if (i >= I) return;
+  const int pb = int(blockIdx.z) * PC;
+  const int pe = min(P, pb + PC);
   for (int64_t o = blockIdx.y; o < O; o += gridDim.y) {
     const TI* xo = X + o * Q * I + i;
     TO* yo = Y + o * int64_t(P) * I + i;
     double x[QMAX];
 #pragma unroll
     for (int q = 0; q < QMAX; ++q) x[q] = (q < Q) ? double(xo[int64_t(q) * I]) : 0.0;
-    for (int p = 0; p < P; ++p) {
-      const double* m = sM + p * Q;
-      double acc = 0.0;
+    for (int p = pb; p < pe; p += 4) {
+      const double* m0 = Mx + int64_t(p) * Q;
+      const double* m1 = Mx + int64_t(min(p + 1, pe - 1)) * Q;
+      const double* m2 = Mx + int64_t(min(p + 2, pe - 1)) * Q;
+      const double* m3 = Mx + int64_t(min(p + 3, pe - 1)) * Q;
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
       for (int q = 0; q < QMAX; ++q)
-        if (q < Q) acc += m[q] * x[q];
-      yo[int64_t(p) * I] = TO(acc);
+        if (q < Q) {
+          a0 += m0[q] * x[q];
+          a1 += m1[q] * x[q];
+          a2 += m2[q] * x[q];
+          a3 += m3[q] * x[q];
+        }
+      yo[int64_t(p) * I] = TO(a0);
+      if (p + 1 < pe) yo[int64_t(p + 1) * I] = TO(a1);
+      if (p + 2 < pe) yo[int64_t(p + 2) * I] = TO(a2);
+      if (p + 3 < pe) yo[int64_t(p + 3) * I] = TO(a3);
     }
   }
 }
@@ -78,15 +92,22 @@ void mix_launch(nsmi_ctx* ctx, const TI* X, TO* Y, const double* Mx, int64_t O, 
   if (O == 0 || I == 0) return;
   ProfScope ps(ctx, cls, 2.0 * double(O) * double(I) * P * Q,
                double(O) * double(I) * (P * sizeof(TO) + Q * sizeof(TI)));
-  dim3 grid(unsigned((I + kMixThreads - 1) / kMixThreads), unsigned(std::min<int64_t>(O, 65535)));
+  // split the outputs into chunks of at least four until the grid holds ~2048 waves
+  const int64_t oy = std::min<int64_t>(O, 65535);
+  const int64_t waves = oy * ((I + 63) / 64);
+  const int64_t want = std::max<int64_t>(1, (2048 + waves - 1) / waves);
+  const int split = int(std::min<int64_t>((P + 3) / 4, want));
+  const int PC = (P + split - 1) / split;
+  const unsigned gz = unsigned((P + PC - 1) / PC);
+  dim3 grid(unsigned((I + kMixThreads - 1) / kMixThreads), unsigned(oy), gz);
   if (Q <= 8)
-    hipLaunchKernelGGL((k_mix<TI, TO, 8>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+    hipLaunchKernelGGL((k_mix<TI, TO, 8>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I, PC);
   else if (Q <= 16)
-    hipLaunchKernelGGL((k_mix<TI, TO, 16>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+    hipLaunchKernelGGL((k_mix<TI, TO, 16>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I, PC);
   else if (Q <= 32)
-    hipLaunchKernelGGL((k_mix<TI, TO, 32>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+    hipLaunchKernelGGL((k_mix<TI, TO, 32>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I, PC);
   else
-    hipLaunchKernelGGL((k_mix<TI, TO, 64>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I);
+    hipLaunchKernelGGL((k_mix<TI, TO, 64>), grid, dim3(kMixThreads), 0, ctx->stream, X, Y, Mx, O, P, Q, I, PC);
   NSMI_LAUNCH_CHECK();
 }
 
