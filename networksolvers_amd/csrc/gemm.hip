@@ -52,6 +52,10 @@ struct GemmArgs {
   int64_t M, N, K, lda, ldb, ldc, sA, sB, sC;
   T alpha, beta;
   int tiles_m, tiles_n;
+  // split-K: workgroup (tile, s) covers K in [s * kc, s * kc + kc); with part set it
+  // writes its raw sum to part[s][M][N] and k_splitk_reduce forms C
+  int64_t kc;
+  T* part;
 };
 
 // XCD-aware tile order: consecutive workgroup ids are dealt round-robin to the
@@ -84,6 +88,8 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
   const int tm = tile / g.tiles_n, tn = tile % g.tiles_n;
   const int64_t m0 = int64_t(tm) * BM, n0 = int64_t(tn) * BN;
   const int64_t bz = blockIdx.z;
+  const int64_t kb = int64_t(blockIdx.y) * g.kc;
+  const int64_t ke = min(g.K, kb + g.kc);
   const T* __restrict__ A = g.A + bz * g.sA;
   const TBm* __restrict__ B = g.B + bz * g.sB;
   TCm* __restrict__ C = g.C + bz * g.sC;
@@ -97,7 +103,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
       const int kk = TA ? (e / BM) : (e % BK);
       const int64_t gm = m0 + mm, gk = k0 + kk;
       T v = T(0);
-      if (gm < g.M && gk < g.K) v = TA ? A[gk * g.lda + gm] : A[gm * g.lda + gk];
+      if (gm < g.M && gk < ke) v = TA ? A[gk * g.lda + gm] : A[gm * g.lda + gk];
       ra[j] = v;
     }
 #pragma unroll
@@ -107,7 +113,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
       const int kk = TB ? (e % BK) : (e / BN);
       const int64_t gn = n0 + nn, gk = k0 + kk;
       T v = T(0);
-      if (gn < g.N && gk < g.K) v = T(TB ? B[gn * g.ldb + gk] : B[gk * g.ldb + gn]);
+      if (gn < g.N && gk < ke) v = T(TB ? B[gn * g.ldb + gk] : B[gk * g.ldb + gn]);
       rb[j] = v;
     }
   };
@@ -134,14 +140,14 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = AccT{0, 0, 0, 0};
 
-  const int64_t nk = (g.K + BK - 1) / BK;
-  load(0);
+  const int64_t nk = (ke - kb + BK - 1) / BK;
+  load(kb);
   store(0);
   __syncthreads();
   const int kl = lane >> 4, ml = lane & 15;
   for (int64_t kt = 0; kt < nk; ++kt) {
     const int buf = int(kt & 1);
-    if (kt + 1 < nk) load((kt + 1) * BK);
+    if (kt + 1 < nk) load(kb + (kt + 1) * BK);
 #pragma unroll
     for (int ks = 0; ks < BK / 4; ++ks) {
       T af[FM], bf[FN];
@@ -165,7 +171,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
       for (int r = 0; r < 4; ++r) {
         const int64_t gm = m0 + wm * WTM + i * 16 + out_row<T>(kl, r);
         const int64_t gn = n0 + wn * WTN + j * 16 + ml;
-        if (gm < g.M && gn < g.N) {
+        if (g.part) {
+          if (gm < g.M && gn < g.N) g.part[(int64_t(blockIdx.y) * g.M + gm) * g.N + gn] = acc[i][j][r];
+        } else if (gm < g.M && gn < g.N) {
           T v = g.alpha * acc[i][j][r];
           if (g.beta != T(0)) v += g.beta * T(C[gm * g.ldc + gn]);
           C[gm * g.ldc + gn] = TCm(v);
@@ -173,21 +181,71 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(GemmArgs<T, TBm, TCm> g) {
       }
 }
 
+// C = alpha * sum_s part[s] + beta * C, the splits added in ascending s (in FP64),
+// so the result does not depend on the order in which the split workgroups ran
+template <typename T, typename TCm>
+__global__ __launch_bounds__(256) void k_splitk_reduce(const T* __restrict__ part, TCm* __restrict__ C, int64_t M,
+                                                       int64_t N, int64_t ldc, int S, T alpha, T beta) {
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= M * N) return;
+  double acc = 0.0;
+  for (int s = 0; s < S; ++s) acc += double(part[int64_t(s) * M * N + e]);
+  const int64_t m = e / N, n = e % N;
+  double v = double(alpha) * acc;
+  if (beta != T(0)) v += double(beta) * double(C[m * ldc + n]);
+  C[m * ldc + n] = TCm(v);
+}
+
+// number of K splits: only for single products whose tile grid leaves most CUs idle
+// and whose K is long; each split keeps at least four K stages
+template <int BM, int BN>
+int splitk_count(const GemmDesc& d, int64_t tiles) {
+  constexpr int BK = (BM == 32 && BN == 32) ? 32 : 16;
+  if (d.batch != 1 || tiles >= 128) return 1;
+  const int64_t stages = (d.K + BK - 1) / BK;
+  const int64_t by_grid = 512 / tiles, by_k = stages / 4;
+  return int(std::max<int64_t>(1, std::min<int64_t>({by_grid, by_k, int64_t(8)})));
+}
+
 template <typename T, typename TBm, typename TCm, int BM, int BN, int ROLE>
 void launch_tile(nsmi_ctx* ctx, const GemmDesc& d, GemmArgs<T, TBm, TCm>& a) {
+  constexpr int BK = (BM == 32 && BN == 32) ? 32 : 16;
   a.tiles_m = int((d.M + BM - 1) / BM);
   a.tiles_n = int((d.N + BN - 1) / BN);
-  dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n), 1, unsigned(d.batch));
+  const int S = splitk_count<BM, BN>(d, int64_t(a.tiles_m) * a.tiles_n);
+  a.kc = S == 1 ? std::max<int64_t>(d.K, 1) : ((d.K + S - 1) / S + BK - 1) / BK * BK;
+  const int splits = int((d.K + a.kc - 1) / a.kc);
+  std::unique_ptr<Scratch> part;
+  a.part = nullptr;
+  if (splits > 1) {
+    part.reset(new Scratch(ctx, sizeof(T) * size_t(splits) * size_t(d.M) * size_t(d.N)));
+    a.part = part->as<T>();
+  }
+  dim3 grid(unsigned(a.tiles_m) * unsigned(a.tiles_n), unsigned(std::max(splits, 1)), unsigned(d.batch));
   dim3 block(256);
+  // with a split the product's timed span runs from the start of the MFMA kernel to
+  // the end of the reduction
+#define NSMI_GEMM_GO(KER)                                                           \
+  do {                                                                              \
+    if (a.part) NSMI_LAUNCH_TIMED_FIRST(ctx, KER, grid, block, a);                  \
+    else NSMI_LAUNCH_TIMED(ctx, KER, grid, block, a);                               \
+  } while (0)
   if (!d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, false, false, ROLE>), grid, block, a);
+    NSMI_GEMM_GO((k_gemm_mfma<T, TBm, TCm, BM, BN, false, false, ROLE>));
   else if (!d.transA && d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, false, true, ROLE>), grid, block, a);
+    NSMI_GEMM_GO((k_gemm_mfma<T, TBm, TCm, BM, BN, false, true, ROLE>));
   else if (d.transA && !d.transB)
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, true, false, ROLE>), grid, block, a);
+    NSMI_GEMM_GO((k_gemm_mfma<T, TBm, TCm, BM, BN, true, false, ROLE>));
   else
-    NSMI_LAUNCH_TIMED(ctx, (k_gemm_mfma<T, TBm, TCm, BM, BN, true, true, ROLE>), grid, block, a);
+    NSMI_GEMM_GO((k_gemm_mfma<T, TBm, TCm, BM, BN, true, true, ROLE>));
+#undef NSMI_GEMM_GO
   NSMI_LAUNCH_CHECK();
+  if (a.part) {
+    const int64_t n = d.M * d.N;
+    NSMI_LAUNCH_TIMED_LAST(ctx, (k_splitk_reduce<T, TCm>), dim3(unsigned((n + 255) / 256)), dim3(256), a.part, a.C,
+                           d.M, d.N, d.ldc, splits, a.alpha, a.beta);
+    NSMI_LAUNCH_CHECK();
+  }
 }
 
 template <typename T, typename TBm, typename TCm, int ROLE>
@@ -209,7 +267,7 @@ void gemm_generic(nsmi_ctx* ctx, const GemmDesc& d, T alpha, const T* A, const T
   require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1, "gemm: bad sizes");
   require(d.tiles_ok(), "gemm: too many tiles");
   if (d.M == 0 || d.N == 0) return;
-  GemmArgs<T> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, alpha, beta, 0, 0};
+  GemmArgs<T> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, alpha, beta, 0, 0, 0, nullptr};
   switch (role) {
     case kRoleHeff1: launch_role<T, T, T, kRoleHeff1>(ctx, d, a); break;
     case kRoleHeff2: launch_role<T, T, T, kRoleHeff2>(ctx, d, a); break;
@@ -238,14 +296,16 @@ void gemm_f32(nsmi_ctx* ctx, const GemmDesc& g, const float* A, const float* B, 
 void gemm_f32_heff1_bd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const double* B, float* C) {
   require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1 && d.tiles_ok(), "gemm: bad sizes");
   if (d.M == 0 || d.N == 0) return;
-  GemmArgs<float, double, float> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0};
+  GemmArgs<float, double, float> a{A,    B,    C,    d.M,  d.N, d.K, d.lda,  d.ldb,
+                                   d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0, 0, nullptr};
   launch_role<float, double, float, kRoleHeff1>(ctx, d, a);
 }
 
 void gemm_f32_heff2_cd(nsmi_ctx* ctx, const GemmDesc& d, const float* A, const float* B, double* C) {
   require(d.M >= 0 && d.N >= 0 && d.K >= 0 && d.batch >= 1 && d.tiles_ok(), "gemm: bad sizes");
   if (d.M == 0 || d.N == 0) return;
-  GemmArgs<float, float, double> a{A, B, C, d.M, d.N, d.K, d.lda, d.ldb, d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0};
+  GemmArgs<float, float, double> a{A,    B,    C,    d.M,  d.N, d.K, d.lda,  d.ldb,
+                                   d.ldc, d.sA, d.sB, d.sC, 1.0f, 0.0f, 0, 0, 0, nullptr};
   launch_role<float, float, double, kRoleHeff2>(ctx, d, a);
 }
 

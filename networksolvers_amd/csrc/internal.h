@@ -189,6 +189,30 @@ struct ProfScope {
     }                                                                                                     \
   } while (0)
 
+// An operation of two launches timed as one: the first carries the start event of a
+// kernel-timed scope, the last its stop event (each recorded by the kernel itself).
+#define NSMI_LAUNCH_TIMED_FIRST(ctx, KERNEL, grid, block, ...)                                      \
+  do {                                                                                              \
+    if ((ctx)->kev_start) {                                                                         \
+      hipExtLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, (ctx)->kev_start, nullptr, 0,    \
+                            __VA_ARGS__);                                                           \
+      (ctx)->kev_start = nullptr;                                                                   \
+    } else {                                                                                        \
+      hipLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, __VA_ARGS__);                       \
+    }                                                                                               \
+  } while (0)
+#define NSMI_LAUNCH_TIMED_LAST(ctx, KERNEL, grid, block, ...)                                       \
+  do {                                                                                              \
+    if ((ctx)->kev_stop && !(ctx)->kev_start) {                                                     \
+      hipExtLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, nullptr, (ctx)->kev_stop, 0,     \
+                            __VA_ARGS__);                                                           \
+      (ctx)->kev_stop = nullptr;                                                                    \
+      (ctx)->kev_used = true;                                                                       \
+    } else {                                                                                        \
+      hipLaunchKernelGGL(KERNEL, grid, block, 0, (ctx)->stream, __VA_ARGS__);                       \
+    }                                                                                               \
+  } while (0)
+
 // ---- dense kernels (gemm.hip) -------------------------------------------
 // Row-major C[b] = alpha * op(A[b]) op(B[b]) + beta * C[b]; op(A) is M x K
 // (A stored K x M when transA), op(B) is K x N (B stored N x K when transB).
