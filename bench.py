@@ -308,6 +308,9 @@ def qn_workload(args):
     t_prof = time.perf_counter() - t0
     stats = ctx.profile_stats()
     ctx.profile(False)
+    table = ctx.tune_table()
+    tuner = {"classes": len(table), "decided": sum(r["choice"] != -1 for r in table),
+             "chose_64": sum(r["choice"] == 64 for r in table)}
     plain = None
     if args.precision != "fp64":
         ctx.sync()
@@ -330,6 +333,7 @@ def qn_workload(args):
                    args.steps, {"fp64": "f64", "fp32": "f32+f64"}[args.precision], config, roof, share)
     if plain:
         line["fp64_plain"] = plain
+    line["block_gemm_tiles"] = tuner
     return line
 
 

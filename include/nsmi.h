@@ -251,6 +251,25 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
                      const int64_t* copy_items, int precision, nsmi_heff** out);
 int nsmi_bsop_run(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y);
 
+/* Tile choice of the block programs' grouped GEMMs (32 x 32 or 64 x 64 output tiles
+ * per workgroup; both give bit-identical results).  No counterpart in the reference:
+ * ITensors contracts each block pair with BLAS.  NSMI_TUNE_AUTO (the default) times
+ * the first launches of each GEMM shape class (transposes, output area, block count,
+ * mean K, share of large blocks) with both tiles and keeps the faster;
+ * NSMI_TUNE_HEURISTIC takes 64 x 64 when at least half the output area lies in
+ * blocks of 48 x 48 or more; NSMI_TUNE_TILE32 / _TILE64 force one tile (64 x 64
+ * only where some block exceeds 32 in a dimension). */
+#define NSMI_TUNE_HEURISTIC 0
+#define NSMI_TUNE_AUTO 1
+#define NSMI_TUNE_TILE32 2
+#define NSMI_TUNE_TILE64 3
+int nsmi_tune_mode(nsmi_ctx* ctx, int mode);
+/* The autotuner's table, one row of 8 int64 per shape class: key, launches issued
+ * with 32 and with 64 tiles, samples completed with each, best time with each in ns
+ * (-1 before a sample), and the chosen tile (32, 64, or -1 while undecided).  Writes
+ * at most cap rows; *n_out receives the number of classes.  Synchronises. */
+int nsmi_tune_table(nsmi_ctx* ctx, int64_t* rows, int cap, int* n_out);
+
 /* Truncated factorisation of a block-diagonal matrix (ITensors factorize of a QN
  * ITensor, inserter.jl:23): sector q is the rows[q] x cols[q] row-major matrix at
  * offsets[q] of Mall.  Every sector is decomposed as nsmi_factorize does; the

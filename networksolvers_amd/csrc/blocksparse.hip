@@ -14,6 +14,7 @@
 // Every offset is checked against its buffer's size when the program is created (x
 // and y when it runs), so a kernel never sees an out-of-range block.
 #include <algorithm>
+#include <map>
 #include <numeric>
 
 #include "internal.h"
@@ -183,12 +184,14 @@ __global__ __launch_bounds__(256) void k_bs_copy(BufTable<T> bt, const CItem* __
 struct BsStep {
   int kind;  // 0 GEMM, 1 LINCOMB, 2 COPY
   bool ta = false, tb = false;
-  int tile = 32;
+  int tile = 32;  // GEMM: the shape heuristic's tile (32 or 64)
   // device arrays (pool blocks) and counts
   void* items = nullptr;
   void* aux = nullptr;    // segments (GEMM) / terms (LINCOMB)
-  void* sched = nullptr;  // tiles (GEMM) / chunks (LINCOMB, COPY)
-  int64_t nsched = 0;
+  void* sched = nullptr;  // tiles (GEMM: the 32 x 32 list, then the 64 x 64 list) / chunks (LINCOMB, COPY)
+  int64_t nsched = 0;     // GEMM: tiles of the 32 x 32 list
+  int64_t nsched64 = 0;   // GEMM: tiles of the 64 x 64 list (0: every block fits one 32 x 32 tile)
+  uint64_t tune_key = 0;  // GEMM: shape class of the step for the tile autotuner
   double flops = 0.0;
 };
 
@@ -240,21 +243,168 @@ void check_range(const BsProgram* p, int buf, int64_t off, int64_t rows, int64_t
   require(last <= p->sizes[buf], std::string("bsop: block outside its buffer in ") + what);
 }
 
+int ilog2(int64_t v) {
+  int r = 0;
+  while (v > 1) {
+    v >>= 1;
+    ++r;
+  }
+  return r;
+}
+
+// shape class of a GEMM step: transposes, log2 of the output area, of the number of
+// blocks and of the mean K per segment, and the share of the area in blocks of at least
+// 48 x 48 (in eighths)
+uint64_t tune_key(const BsStep& s, const std::vector<GItem>& its, const std::vector<GSeg>& sg, int64_t all,
+                  int64_t big) {
+  int64_t ksum = 0;
+  for (const GSeg& g : sg) ksum += g.K;
+  const int64_t kmean = sg.empty() ? 0 : ksum / int64_t(sg.size());
+  const uint64_t share = all > 0 ? uint64_t((8 * big) / all) : 0;
+  return uint64_t(s.ta) | (uint64_t(s.tb) << 1) | (uint64_t(std::min(ilog2(all + 1), 63)) << 2) |
+         (uint64_t(std::min(ilog2(int64_t(its.size()) + 1), 31)) << 8) |
+         (uint64_t(std::min(ilog2(kmean + 1), 31)) << 13) | (std::min<uint64_t>(share, 8) << 18);
+}
+
 template <typename T, int BM, int BN>
-void launch_gemm(nsmi_ctx* ctx, const BufTable<T>& bt, const BsStep& s) {
-  const dim3 grid(unsigned(s.nsched)), block(256);
+void launch_gemm(nsmi_ctx* ctx, const BufTable<T>& bt, const BsStep& s, hipEvent_t ea, hipEvent_t eb) {
   const GItem* it = static_cast<const GItem*>(s.items);
   const GSeg* sg = static_cast<const GSeg*>(s.aux);
-  const GTile* tl = static_cast<const GTile*>(s.sched);
+  const GTile* tl = static_cast<const GTile*>(s.sched) + (BM == 64 ? s.nsched : 0);
+  const dim3 grid(unsigned(BM == 64 ? s.nsched64 : s.nsched)), block(256);
+#define NSMI_BS_GO(KER)                                                                      \
+  do {                                                                                       \
+    if (ea)                                                                                  \
+      hipExtLaunchKernelGGL(KER, grid, block, 0, ctx->stream, ea, eb, 0, bt, it, sg, tl);    \
+    else                                                                                     \
+      hipLaunchKernelGGL(KER, grid, block, 0, ctx->stream, bt, it, sg, tl);                  \
+  } while (0)
   if (!s.ta && !s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, false, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    NSMI_BS_GO((k_bs_gemm<T, BM, BN, false, false>));
   else if (!s.ta && s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, false, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    NSMI_BS_GO((k_bs_gemm<T, BM, BN, false, true>));
   else if (s.ta && !s.tb)
-    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, true, false>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    NSMI_BS_GO((k_bs_gemm<T, BM, BN, true, false>));
   else
-    hipLaunchKernelGGL((k_bs_gemm<T, BM, BN, true, true>), grid, block, 0, ctx->stream, bt, it, sg, tl);
+    NSMI_BS_GO((k_bs_gemm<T, BM, BN, true, true>));
+#undef NSMI_BS_GO
   NSMI_LAUNCH_CHECK();
+}
+
+// ---- tile autotuner -------------------------------------------------------
+// Each GEMM step has a shape class (tune_key).  The first launches of a class alternate
+// between the 32 x 32 and the 64 x 64 tile list, each timed by events on the launch
+// itself; once both have kTuneSamples completed samples, the class keeps the tile with
+// the shorter best time.  Event results are collected without waiting (hipEventQuery)
+// at later launches.  Both lists give bit-identical results: every output element sums
+// its segments and K in the same order, whatever the tile.
+constexpr int kTuneSamples = 3;
+constexpr size_t kTuneMaxPending = 256;
+
+struct TuneEntry {
+  int issued[2] = {0, 0};
+  int done[2] = {0, 0};
+  float best[2] = {0.f, 0.f};
+  int choice = -1;
+};
+struct TunePending {
+  hipEvent_t a, b;
+  uint64_t key;
+  int cand;
+};
+
+}  // namespace
+
+struct Tuner {
+  std::map<uint64_t, TuneEntry> table;
+  std::vector<TunePending> pending;
+  std::vector<hipEvent_t> spare;
+};
+
+void tuner_free(nsmi_ctx* ctx) {
+  Tuner* t = ctx->tuner;
+  if (!t) return;
+  for (auto& q : t->pending) {
+    (void)hipEventSynchronize(q.b);
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  for (auto e : t->spare) (void)hipEventDestroy(e);
+  delete t;
+  ctx->tuner = nullptr;
+}
+
+namespace {
+
+Tuner* tuner_of(nsmi_ctx* ctx) {
+  if (!ctx->tuner) ctx->tuner = new Tuner();
+  return ctx->tuner;
+}
+
+hipEvent_t tune_event(Tuner* t) {
+  if (!t->spare.empty()) {
+    hipEvent_t e = t->spare.back();
+    t->spare.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  NSMI_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void tune_poll(Tuner* t) {
+  size_t keep = 0;
+  for (size_t i = 0; i < t->pending.size(); ++i) {
+    TunePending q = t->pending[i];
+    const hipError_t st = hipEventQuery(q.b);
+    if (st == hipErrorNotReady) {
+      t->pending[keep++] = q;
+      continue;
+    }
+    float ms = 0.f;
+    if (st == hipSuccess && hipEventElapsedTime(&ms, q.a, q.b) == hipSuccess) {
+      TuneEntry& e = t->table[q.key];
+      const int c = q.cand;
+      e.best[c] = e.done[c] == 0 ? ms : std::min(e.best[c], ms);
+      e.done[c] += 1;
+      if (e.choice < 0 && e.done[0] >= kTuneSamples && e.done[1] >= kTuneSamples)
+        e.choice = e.best[1] < e.best[0] ? 1 : 0;
+    }
+    t->spare.push_back(q.a);
+    t->spare.push_back(q.b);
+  }
+  t->pending.resize(keep);
+}
+
+template <typename T>
+void launch_tuned(nsmi_ctx* ctx, const BufTable<T>& bt, const BsStep& s, bool lo) {
+  int cand = s.tile == 64 ? 1 : 0;  // NSMI_TUNE_HEURISTIC
+  hipEvent_t ea = nullptr, eb = nullptr;
+  uint64_t key = s.tune_key | (lo ? (uint64_t(1) << 63) : 0);
+  if (s.nsched64 == 0) {
+    cand = 0;
+  } else if (ctx->tune_mode == NSMI_TUNE_TILE32) {
+    cand = 0;
+  } else if (ctx->tune_mode == NSMI_TUNE_TILE64) {
+    cand = 1;
+  } else if (ctx->tune_mode == NSMI_TUNE_AUTO) {
+    Tuner* t = tuner_of(ctx);
+    if (!t->pending.empty()) tune_poll(t);
+    TuneEntry& e = t->table[key];
+    if (e.choice >= 0) {
+      cand = e.choice;
+    } else if (std::min(e.issued[0], e.issued[1]) < kTuneSamples && t->pending.size() < kTuneMaxPending) {
+      cand = e.issued[0] <= e.issued[1] ? 0 : 1;
+      e.issued[cand] += 1;
+      ea = tune_event(t);
+      eb = tune_event(t);
+      t->pending.push_back(TunePending{ea, eb, key, cand});
+    }
+  }
+  if (cand == 1)
+    launch_gemm<T, 64, 64>(ctx, bt, s, ea, eb);
+  else
+    launch_gemm<T, 32, 32>(ctx, bt, s, ea, eb);
 }
 
 template <typename T>
@@ -262,10 +412,7 @@ void run_steps(nsmi_ctx* ctx, const BsProgram* p, const BufTable<T>& bt) {
   for (const BsStep& s : p->steps) {
     if (s.nsched == 0) continue;
     if (s.kind == 0) {
-      if (s.tile == 64)
-        launch_gemm<T, 64, 64>(ctx, bt, s);
-      else
-        launch_gemm<T, 32, 32>(ctx, bt, s);
+      launch_tuned<T>(ctx, bt, s, p->lo);
     } else if (s.kind == 1) {
       hipLaunchKernelGGL(k_bs_lincomb<T>, dim3(unsigned(s.nsched)), dim3(256), 0, ctx->stream, bt,
                          static_cast<const LItem*>(s.items), static_cast<const LTerm*>(s.aux),
@@ -415,17 +562,25 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
         if (it.M >= 48 && it.N >= 48) big += it.M * it.N;
       }
       s.tile = (2 * big >= all && all > 0) ? 64 : 32;
-      const int T = s.tile;
-      for (size_t k = 0; k < its.size(); ++k) {
-        const int64_t tm = (its[k].M + T - 1) / T, tn = (its[k].N + T - 1) / T;
-        for (int64_t a = 0; a < tm; ++a)
-          for (int64_t b = 0; b < tn; ++b) tl.push_back(GTile{int(k), int(a), int(b), 0});
+      // both tile lists, 32 x 32 first; the 64 x 64 list only when some block exceeds 32
+      bool wide = false;
+      for (const GItem& g : its) wide = wide || g.M > 32 || g.N > 32;
+      for (int T : {32, 64}) {
+        if (T == 64 && !wide) break;
+        for (size_t k = 0; k < its.size(); ++k) {
+          const int64_t tm = (its[k].M + T - 1) / T, tn = (its[k].N + T - 1) / T;
+          for (int64_t a = 0; a < tm; ++a)
+            for (int64_t b = 0; b < tn; ++b) tl.push_back(GTile{int(k), int(a), int(b), 0});
+        }
+        if (T == 32) s.nsched = int64_t(tl.size());
       }
+      s.nsched64 = int64_t(tl.size()) - s.nsched;
+      if (!wide) s.tile = 32;
       require(tl.size() < (size_t(1) << 31), "bsop: too many tiles");
       s.items = upload_vec(ctx, its);
       s.aux = upload_vec(ctx, sg);
       s.sched = upload_vec(ctx, tl);
-      s.nsched = int64_t(tl.size());
+      s.tune_key = tune_key(s, its, sg, all, big);
     } else if (s.kind == 1) {
       std::vector<LItem> its;
       std::vector<LTerm> tm;
@@ -490,6 +645,44 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
     return NSMI_ERR_INVALID;
   }
   return NSMI_OK;
+}
+
+int nsmi_tune_mode(nsmi_ctx* ctx, int mode) {
+  if (!ctx) return NSMI_ERR_INVALID;
+  API_BEGIN
+  require(mode == NSMI_TUNE_HEURISTIC || mode == NSMI_TUNE_AUTO || mode == NSMI_TUNE_TILE32 ||
+              mode == NSMI_TUNE_TILE64,
+          "tune_mode: unknown mode");
+  ctx->tune_mode = mode;
+  API_END(ctx)
+}
+
+int nsmi_tune_table(nsmi_ctx* ctx, int64_t* rows, int cap, int* n_out) {
+  if (!ctx || !n_out || cap < 0 || (cap > 0 && !rows)) return NSMI_ERR_INVALID;
+  API_BEGIN
+  *n_out = 0;
+  Tuner* t = ctx->tuner;
+  if (!t) return NSMI_OK;
+  NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  tune_poll(t);
+  int n = 0;
+  for (const auto& kv : t->table) {
+    if (n < cap) {
+      int64_t* r = rows + 8 * n;
+      const TuneEntry& e = kv.second;
+      r[0] = int64_t(kv.first);
+      r[1] = e.issued[0];
+      r[2] = e.issued[1];
+      r[3] = e.done[0];
+      r[4] = e.done[1];
+      r[5] = e.done[0] ? int64_t(double(e.best[0]) * 1e6) : -1;
+      r[6] = e.done[1] ? int64_t(double(e.best[1]) * 1e6) : -1;
+      r[7] = e.choice < 0 ? -1 : (e.choice == 1 ? 64 : 32);
+    }
+    ++n;
+  }
+  *n_out = n;
+  API_END(ctx)
 }
 
 int nsmi_bsop_run(nsmi_ctx* ctx, nsmi_heff* op, const nsmi_tensor* x, nsmi_tensor* y) {

@@ -297,6 +297,7 @@ int nsmi_ctx_destroy(nsmi_ctx* ctx) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : ctx->spare_events) (void)hipEventDestroy(e);
+  nsmi::tuner_free(ctx);
   for (size_t j = 0; j < ctx->aux_stream.size(); ++j) {
     (void)hipStreamSynchronize(ctx->aux_stream[j]);
     if (j < ctx->aux_blas.size()) rocblas_destroy_handle(ctx->aux_blas[j]);

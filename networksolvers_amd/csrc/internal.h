@@ -59,6 +59,8 @@ class Pool {
   std::map<void*, size_t> live_;
 };
 
+struct Tuner;  // tile autotuner of the block-program GEMMs (blocksparse.hip)
+
 struct ProfRec {
   hipEvent_t a, b;
   int cls;
@@ -108,6 +110,9 @@ struct nsmi_ctx {
   std::vector<nsmi::ProfRec> recs;
   std::vector<hipEvent_t> spare_events;
   nsmi_profile_stats stats[NSMI_K_COUNT] = {};
+  // tile choice of the block-program GEMMs (NSMI_TUNE_*) and the autotuner's table
+  int tune_mode = NSMI_TUNE_AUTO;
+  nsmi::Tuner* tuner = nullptr;
 };
 
 namespace nsmi {
@@ -348,6 +353,7 @@ nsmi_heff* heff_wrap_net(NetProgram* p);  // region.hip: an nsmi_heff that runs 
 struct BsProgram;
 void bs_run(nsmi_ctx* ctx, BsProgram* p, const double* x, double* y);
 void bs_release(nsmi_ctx* ctx, BsProgram* p);
+void tuner_free(nsmi_ctx* ctx);  // the autotuner's events and table (context teardown)
 nsmi_heff* heff_wrap_bs(BsProgram* p, int64_t n);  // region.hip; n < 0: not a square operator
 BsProgram* heff_bs_program(nsmi_heff* op);
 

@@ -93,6 +93,28 @@ class Context:
         """Record one launch in ``every`` per kernel class while profiling is on."""
         self.check(self.lib.nsmi_profile_sampling(self.h, int(every)))
 
+    TUNE_MODES = {"heuristic": 0, "auto": 1, "tile32": 2, "tile64": 3}
+
+    def tune_mode(self, mode: str = "auto"):
+        """Tile choice of the block-program GEMMs: "auto" (the autotuner, default),
+        "heuristic", or a forced "tile32" / "tile64"; every choice gives the same bits."""
+        self.check(self.lib.nsmi_tune_mode(self.h, self.TUNE_MODES[mode]))
+
+    def tune_table(self):
+        """The autotuner's classes: key, launches issued and samples completed per
+        tile, best time per tile in ns (-1 before a sample) and the chosen tile."""
+        n = C.c_int(0)
+        self.check(self.lib.nsmi_tune_table(self.h, None, 0, C.byref(n)))
+        cap = n.value
+        buf = (C.c_int64 * (8 * max(cap, 1)))()
+        self.check(self.lib.nsmi_tune_table(self.h, buf, cap, C.byref(n)))
+        rows = []
+        for i in range(min(cap, n.value)):
+            r = buf[8 * i:8 * i + 8]
+            rows.append({"key": r[0], "issued": (r[1], r[2]), "samples": (r[3], r[4]),
+                         "best_ns": (r[5], r[6]), "choice": r[7]})
+        return rows
+
     def profile_stats(self):
         out = {}
         for name, k in _lib.K_CLASSES.items():
