@@ -520,6 +520,10 @@ def summarize_rocprof(root, window_log=None, top=25):
             out.append("-- dispatches inside the timed window (name, calls, total_ms, avg_us)")
             for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
                 out.append(f"{name[:150]}, {c}, {t / 1e6:.3f}, {t / c / 1e3:.3f}")
+            out.append("-- every MFMA GEMM and split-K reduction inside the timed window (name, calls, total_ms, avg_us)")
+            for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+                if "k_gemm_mfma" in name or "k_splitk_reduce" in name:
+                    out.append(f"{name[:150]}, {c}, {t / 1e6:.3f}, {t / c / 1e3:.3f}")
     for f in glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True):
         agg = collections.defaultdict(lambda: [0, 0.0, 0.0])
         aggw = collections.defaultdict(lambda: [0, 0.0])
