@@ -14,7 +14,7 @@
 //     p = tau_j A22 v,  w = p - (tau_j/2)(p.v) v,  A22 -= v w^T + w v^T,  d_j = A(j, j)
 // The output has LAPACK's layout -- d, e, tau, and in the lower triangle of the
 // column-major view the diagonal d, the subdiagonal e and v_j(2:) in A(j+2:n, j)
-// -- which rocsolver_dstedc and rocsolver_dormtr consume.
+// -- which rocsolver_dstedc and the back-transformation apply_q_lower consume.
 //
 // Distribution: workgroup g owns the rows i = g (mod G); row i sits in registers
 // (of one wave, or of one 16-lane group; see the two kernels).  Step j:
@@ -564,6 +564,127 @@ __global__ __launch_bounds__(256) void k_jacobi_eig(const JacobiItem* __restrict
   for (int i = tid; i < n; i += 256) it.w[i] = A[i][i];
 }
 
+
+// ---- back-transformation Q . Z (Q = H_0 H_1 ... H_{n-2} from the reduction) ---------
+// Compact-WY form with the UT transform (Joffrain et al., "Accumulating Householder
+// transformations, revisited", ACM TOMS 32, 2006): Q = I - V S^{-1} V^T, with V the
+// n x (n-1) matrix of reflector vectors and S upper triangular, S_ii = 1 / tau_i,
+// S_ij = v_i . v_j (i < j).  Hence Q Z = Z - V X with S X = V^T Z: two GEMMs for
+// V^T Z and V^T V, one triangular solve, one GEMM for the update.  A reflector with
+// tau = 0 is the identity: its row of V^T is zero and S_ii = 1, so it drops out.
+
+// Vt (m x n, row-major; m = n - 1): row i is v_i -- zero up to i, 1 at i + 1, then the
+// stored A(i+2:n, i), which in the row-major buffer of the column-major view is
+// rho[i n + r]
+__global__ void k_build_vt(const double* __restrict__ rho, const double* __restrict__ tau, int n,
+                           double* __restrict__ Vt) {
+  const int64_t m = n - 1;
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= m * n) return;
+  const int64_t i = e / n, r = e % n;
+  double v = 0.0;
+  if (tau[i] != 0.0) v = r <= i ? 0.0 : (r == i + 1 ? 1.0 : rho[i * n + r]);
+  Vt[e] = v;
+}
+
+__device__ inline double lane_bcast(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+// X <- S^{-1} X in place (X: m x k row-major), S from G = V^T V (strict upper part
+// read) and tau.  One workgroup per kUtCols columns of X, held in LDS; back
+// substitution by blocks of kUtRows rows from the bottom: wave 0 solves the block's
+// triangle in registers (lane a = row a of the block, the pivot broadcast by
+// readlane), then all waves subtract the block's contribution from the rows above.
+constexpr int kUtCols = 4, kUtRows = 32, kUtMaxM = 1023;
+__global__ __launch_bounds__(256) void k_ut_solve(const double* __restrict__ G, const double* __restrict__ tau,
+                                                  double* __restrict__ X, int m, int k) {
+  __shared__ double xs[(kUtMaxM + 1) * kUtCols];
+  __shared__ double sd[kUtRows][kUtRows + 1];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * kUtCols;
+  const int nc = min(kUtCols, k - c0);
+  for (int e = tid; e < m * kUtCols; e += 256) {
+    const int i = e / kUtCols, c = e % kUtCols;
+    xs[e] = c < nc ? X[int64_t(i) * k + c0 + c] : 0.0;
+  }
+  __syncthreads();
+  for (int r1 = m; r1 > 0; r1 -= kUtRows) {
+    const int r0 = max(0, r1 - kUtRows), bs = r1 - r0;
+    for (int e = tid; e < kUtRows * kUtRows; e += 256) {
+      const int a = e / kUtRows, b = e % kUtRows;
+      sd[a][b] = (a < bs && b < bs && b > a) ? G[int64_t(r0 + a) * m + r0 + b] : 0.0;
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const int a = tid;
+      double xa[kUtCols];
+#pragma unroll
+      for (int c = 0; c < kUtCols; ++c) xa[c] = a < bs ? xs[(r0 + a) * kUtCols + c] : 0.0;
+      for (int ii = bs - 1; ii >= 0; --ii) {
+        const double t = tau[r0 + ii];
+        const double sc = t == 0.0 ? 1.0 : t;  // x_ii / S_ii
+        if (a == ii) {
+#pragma unroll
+          for (int c = 0; c < kUtCols; ++c) xa[c] *= sc;
+        }
+        const double s = a < ii ? sd[a][ii] : 0.0;
+#pragma unroll
+        for (int c = 0; c < kUtCols; ++c) xa[c] -= s * lane_bcast(xa[c], ii);
+      }
+      if (a < bs) {
+#pragma unroll
+        for (int c = 0; c < kUtCols; ++c) xs[(r0 + a) * kUtCols + c] = xa[c];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < r0 * kUtCols; e += 256) {
+      const int j = e / kUtCols, c = e % kUtCols;
+      const double* g = G + int64_t(j) * m + r0;
+      double acc = 0.0;
+      for (int b = 0; b < bs; ++b) acc += g[b] * xs[(r0 + b) * kUtCols + c];
+      xs[e] -= acc;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * kUtCols; e += 256) {
+    const int i = e / kUtCols, c = e % kUtCols;
+    if (c < nc) X[int64_t(i) * k + c0 + c] = xs[e];
+  }
+}
+
+}  // namespace
+
+void apply_q_lower(nsmi_ctx* ctx, const double* rho, const double* tau, int n, double* Zr, int k) {
+  require(n >= 2 && n <= kUtMaxM + 1 && k >= 0, "apply_q_lower: n must be in [2, 1024]");
+  if (k == 0) return;
+  const int m = n - 1;
+  Scratch vt(ctx, sizeof(double) * size_t(m) * n), g(ctx, sizeof(double) * size_t(m) * m),
+      x(ctx, sizeof(double) * size_t(m) * k);
+  const int64_t nv = int64_t(m) * n;
+  hipLaunchKernelGGL(k_build_vt, dim3(unsigned((nv + 255) / 256)), dim3(256), 0, ctx->stream, rho, tau, n,
+                     vt.as<double>());
+  NSMI_LAUNCH_CHECK();
+  GemmDesc dw{};  // X = Vt . Zr^T  (m x k)
+  dw.M = m; dw.N = k; dw.K = n; dw.transA = false; dw.transB = true;
+  dw.lda = n; dw.ldb = n; dw.ldc = k; dw.batch = 1;
+  gemm_f64(ctx, dw, 1.0, vt.as<double>(), Zr, 0.0, x.as<double>(), kRoleFact);
+  GemmDesc dg{};  // G = Vt . Vt^T  (m x m)
+  dg.M = m; dg.N = m; dg.K = n; dg.transA = false; dg.transB = true;
+  dg.lda = n; dg.ldb = n; dg.ldc = m; dg.batch = 1;
+  gemm_f64(ctx, dg, 1.0, vt.as<double>(), vt.as<double>(), 0.0, g.as<double>(), kRoleFact);
+  hipLaunchKernelGGL(k_ut_solve, dim3(unsigned((k + kUtCols - 1) / kUtCols)), dim3(256), 0, ctx->stream,
+                     g.as<double>(), tau, x.as<double>(), m, k);
+  NSMI_LAUNCH_CHECK();
+  GemmDesc du{};  // Zr -= X^T . Vt  (k x n)
+  du.M = k; du.N = n; du.K = m; du.transA = true; du.transB = false;
+  du.lda = k; du.ldb = n; du.ldc = n; du.batch = 1;
+  gemm_f64(ctx, du, -1.0, x.as<double>(), vt.as<double>(), 1.0, Zr, kRoleFact);
+}
+
+namespace {
 }  // namespace
 
 int sytrd_coop_groups(int n) {

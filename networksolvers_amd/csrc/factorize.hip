@@ -3,7 +3,7 @@
 //
 // Both routes diagonalise rho = M M^T (isometry on the rows; M^T M on the
 // columns) formed by MFMA GEMM: batched Jacobi for small rho, the one-launch
-// tridiagonal reduction with rocSOLVER dstedc / dormtr up to 1024, dsyevd above.
+// tridiagonal reduction with rocSOLVER dstedc and the compact-WY back-transformation up to 1024, dsyevd above.
 // "eigen" sorts the spectrum by decreasing |lambda|; "svd" takes P = s^2 =
 // max(lambda, 0) in decreasing order (its P is resolved to eps * P_max
 // absolutely, not relatively as by an SVD of M).  NDTensors truncate! is
@@ -407,16 +407,12 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
     NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
     if (f.tri) {
       // kept eigenvectors of T (columns order[0..k) of z), then Q . Zk with the reflectors
-      ProfScope ps2(ctx, NSMI_K_FACT_SOLVE, 4.0 * double(iso) * iso * k, 0.0);
+      ProfScope ps2(ctx, NSMI_K_FACT_SOLVE, 2.0 * double(iso) * iso * (2.0 * k + iso), 0.0);
       zk.reset(new Scratch(ctx, sizeof(double) * size_t(k) * iso));
       hipLaunchKernelGGL(k_gather_rows, dim3(blocks(k * iso)), dim3(256), 0, ctx->stream, f.z->as<double>(), iso,
                          vidx, int(k), iso, (const double*)nullptr, (const double*)nullptr, zk->as<double>());
       NSMI_LAUNCH_CHECK();
-      if (iso > 1)
-        check_rocsolver(rocsolver_dormtr(ctx->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                         rocblas_int(iso), rocblas_int(k), f.rho->as<double>(), rocblas_int(iso),
-                                         f.tau->as<double>(), zk->as<double>(), rocblas_int(iso)),
-                        "rocsolver_dormtr");
+      if (iso > 1) apply_q_lower(ctx, f.rho->as<double>(), f.tau->as<double>(), int(iso), zk->as<double>(), int(k));
       vecs = zk->as<double>();
       vidx = nullptr;
     }

@@ -320,6 +320,10 @@ struct SytrdRun {
 };
 bool sytrd_launch(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau, SytrdRun& run);
 void sytrd_check(const SytrdRun& run);
+// Zr <- (Q Z) with Q = H_0 ... H_{n-2} the reflectors of sytrd_launch's output (rho,
+// tau; n <= 1024) and Z = Zr^T (Zr: k x n row-major, i.e. Z column-major n x k): what
+// LAPACK dormtr("L", "L", "N") computes, as GEMMs and one triangular solve.
+void apply_q_lower(nsmi_ctx* ctx, const double* rho, const double* tau, int n, double* Zr, int k);
 // Batched Jacobi eigensolver for small symmetric matrices (one workgroup each, LDS
 // resident): eigenvectors overwrite a (column-major, dsyevd layout), eigenvalues
 // (unsorted) go to w.  Enqueues only: *unconverged (host memory) receives the
