@@ -231,7 +231,8 @@ def test_factorize(ctx, route, trailing, shape, cutoff, maxdim):
                                          (128, 130, None), (129, 140, 100), (150, 150, None), (256, 260, 128),
                                          (257, 300, 128), (384, 390, 200), (385, 400, 200), (448, 450, 220), (449, 460, 220),
                                          (512, 520, 256),
-                                         (600, 620, 300), (700, 700, 350), (1100, 1000, 500), (2100, 2100, 700)])
+                                         (600, 620, 300), (700, 700, 350), (1024, 1030, 500), (1100, 1000, 500),
+                                         (1025, 1030, 500), (2100, 2100, 700)])
 def test_factorize_eigensolver_paths(ctx, m, n, maxdim):
     """Gram sizes across every eigensolver path, with a graded spectrum: the
     batched Jacobi solver for small Gram matrices, every plan of the one-launch
@@ -255,6 +256,32 @@ def test_factorize_eigensolver_paths(ctx, m, n, maxdim):
     np.testing.assert_allclose(Ug @ Cg, Uo @ Co, atol=1e-12)
     good = io.spectrum[:info.kept] >= 1e-6 * io.spectrum[0]
     np.testing.assert_allclose(Ug[:, good], Uo[:, good], atol=1e-9)
+
+
+@pytest.mark.parametrize("blocks", [[(60, 70), (50, 40)], [(30, 30), (1, 5), (90, 80), (12, 12)]])
+def test_factorize_block_diagonal(ctx, blocks):
+    """A block-diagonal theta: its Gram matrix is block-diagonal, so the tridiagonal
+    reduction meets columns that are already zero below the subdiagonal (reflectors
+    with tau = 0, the identity), which the back-transformation must drop; the
+    spectrum and the vectors still match the oracle."""
+    import networksolvers_amd as ns
+    rng = np.random.default_rng(sum(a * b for a, b in blocks))
+    m, n = sum(a for a, _ in blocks), sum(b for _, b in blocks)
+    M = np.zeros((m, n))
+    r0 = c0 = 0
+    for i, (a, b) in enumerate(blocks):
+        r = min(a, b)
+        U0, _ = np.linalg.qr(rng.standard_normal((a, r)))
+        V0, _ = np.linalg.qr(rng.standard_normal((b, r)))
+        M[r0:r0 + a, c0:c0 + b] = (U0 * np.exp(-np.arange(r) / 9.0 - 0.37 * i)) @ V0.T
+        r0, c0 = r0 + a, c0 + b
+    U, C, info = ns.factorize(ctx.upload(M.reshape(m, 1, 1, n)), 2, False, cutoff=1e-10, which_decomp="eigen")
+    Uo, Co, io = linalg.factorize(M, cutoff=1e-10, which_decomp="eigen")
+    assert info.kept == io.kept
+    assert abs(info.truncerr - io.truncerr) < 1e-13 + 1e-8 * io.truncerr
+    Ug, Cg = U.numpy().reshape(m, -1), C.numpy().reshape(-1, n)
+    np.testing.assert_allclose(Ug.T @ Ug, np.eye(info.kept), atol=1e-12)
+    np.testing.assert_allclose(Ug @ Cg, Uo @ Co, atol=1e-12)
 
 
 @pytest.mark.parametrize("trailing", [False, True])
