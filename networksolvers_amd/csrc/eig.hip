@@ -594,15 +594,19 @@ __device__ inline double lane_bcast(double v, int lane) {
 }
 
 // X <- S^{-1} X in place (X: m x k row-major), S from G = V^T V (strict upper part
-// read) and tau.  One workgroup per kUtCols columns of X, held in LDS; back
-// substitution by blocks of kUtRows rows from the bottom: wave 0 solves the block's
-// triangle in registers (lane a = row a of the block, the pivot broadcast by
-// readlane), then all waves subtract the block's contribution from the rows above.
+// read; G is symmetric, so G[j][i] is read as G[i][j] wherever that makes the reads
+// contiguous across lanes) and tau.  One workgroup per kUtCols columns of X, held in
+// LDS with tau; back substitution by blocks of kUtRows rows from the bottom: wave 0
+// solves the block's triangle in registers (lane a = row a of the block with its row
+// of S, fully unrolled, each pivot broadcast by readlane), then all waves subtract the
+// block's contribution from the rows above (one row of X per thread, coalesced reads
+// of G, all of a block's loads issued before the first is used).  Wave 0 loads the
+// next block's rows of S before that update, so their latency overlaps it.
 constexpr int kUtCols = 4, kUtRows = 32, kUtMaxM = 1023;
 __global__ __launch_bounds__(256) void k_ut_solve(const double* __restrict__ G, const double* __restrict__ tau,
                                                   double* __restrict__ X, int m, int k) {
   __shared__ double xs[(kUtMaxM + 1) * kUtCols];
-  __shared__ double sd[kUtRows][kUtRows + 1];
+  __shared__ double ts[kUtMaxM + 1];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * kUtCols;
   const int nc = min(kUtCols, k - c0);
@@ -610,29 +614,37 @@ __global__ __launch_bounds__(256) void k_ut_solve(const double* __restrict__ G, 
     const int i = e / kUtCols, c = e % kUtCols;
     xs[e] = c < nc ? X[int64_t(i) * k + c0 + c] : 0.0;
   }
+  for (int i = tid; i < m; i += 256) ts[i] = tau[i];
+  // row a = tid of the block's strict upper triangle: S[r0 + a][r0 + b] = G[r0 + b][r0 + a], b > a
+  double srow[kUtRows];
+  auto load_srow = [&](int r0, int bs) {
+#pragma unroll
+    for (int b = 0; b < kUtRows; ++b)
+      srow[b] = (tid < bs && b < bs && b > tid) ? G[int64_t(r0 + b) * m + r0 + tid] : 0.0;
+  };
+  int r1 = m, r0 = max(0, m - kUtRows);
+  if (tid < 64) load_srow(r0, r1 - r0);
   __syncthreads();
-  for (int r1 = m; r1 > 0; r1 -= kUtRows) {
-    const int r0 = max(0, r1 - kUtRows), bs = r1 - r0;
-    for (int e = tid; e < kUtRows * kUtRows; e += 256) {
-      const int a = e / kUtRows, b = e % kUtRows;
-      sd[a][b] = (a < bs && b < bs && b > a) ? G[int64_t(r0 + a) * m + r0 + b] : 0.0;
-    }
-    __syncthreads();
+  while (r1 > 0) {
+    const int bs = r1 - r0;
     if (tid < 64) {
       const int a = tid;
       double xa[kUtCols];
+      const double t = a < bs ? ts[r0 + a] : 1.0;
+      const double sc = t == 0.0 ? 1.0 : t;  // x_a / S_aa
 #pragma unroll
       for (int c = 0; c < kUtCols; ++c) xa[c] = a < bs ? xs[(r0 + a) * kUtCols + c] : 0.0;
-      for (int ii = bs - 1; ii >= 0; --ii) {
-        const double t = tau[r0 + ii];
-        const double sc = t == 0.0 ? 1.0 : t;  // x_ii / S_ii
-        if (a == ii) {
 #pragma unroll
-          for (int c = 0; c < kUtCols; ++c) xa[c] *= sc;
+      for (int ii = kUtRows - 1; ii >= 0; --ii) {
+        if (ii < bs) {
+          if (a == ii) {
+#pragma unroll
+            for (int c = 0; c < kUtCols; ++c) xa[c] *= sc;
+          }
+          // srow[ii] is zero for every lane a >= ii (strict upper part)
+#pragma unroll
+          for (int c = 0; c < kUtCols; ++c) xa[c] -= srow[ii] * lane_bcast(xa[c], ii);
         }
-        const double s = a < ii ? sd[a][ii] : 0.0;
-#pragma unroll
-        for (int c = 0; c < kUtCols; ++c) xa[c] -= s * lane_bcast(xa[c], ii);
       }
       if (a < bs) {
 #pragma unroll
@@ -640,14 +652,23 @@ __global__ __launch_bounds__(256) void k_ut_solve(const double* __restrict__ G, 
       }
     }
     __syncthreads();
-    for (int e = tid; e < r0 * kUtCols; e += 256) {
-      const int j = e / kUtCols, c = e % kUtCols;
-      const double* g = G + int64_t(j) * m + r0;
-      double acc = 0.0;
-      for (int b = 0; b < bs; ++b) acc += g[b] * xs[(r0 + b) * kUtCols + c];
-      xs[e] -= acc;
+    const int n1 = r0, n0 = max(0, r0 - kUtRows);
+    if (tid < 64 && n1 > 0) load_srow(n0, n1 - n0);
+    for (int j = tid; j < r0; j += 256) {
+      double g[kUtRows];
+#pragma unroll
+      for (int b = 0; b < kUtRows; ++b) g[b] = b < bs ? G[int64_t(r0 + b) * m + j] : 0.0;
+      double acc[kUtCols] = {};
+#pragma unroll
+      for (int b = 0; b < kUtRows; ++b)
+#pragma unroll
+        for (int c = 0; c < kUtCols; ++c) acc[c] += g[b] * xs[(r0 + min(b, bs - 1)) * kUtCols + c];
+#pragma unroll
+      for (int c = 0; c < kUtCols; ++c) xs[j * kUtCols + c] -= acc[c];
     }
     __syncthreads();
+    r1 = n1;
+    r0 = n0;
   }
   for (int e = tid; e < m * kUtCols; e += 256) {
     const int i = e / kUtCols, c = e % kUtCols;
