@@ -705,9 +705,6 @@ void apply_q_lower(nsmi_ctx* ctx, const double* rho, const double* tau, int n, d
   gemm_f64(ctx, du, -1.0, x.as<double>(), vt.as<double>(), 1.0, Zr, kRoleFact);
 }
 
-namespace {
-}  // namespace
-
 int sytrd_coop_groups(int n) {
   if (n < 2 || n > kSytrdCoopMax) return 0;
   const TriPlan p = tri_plan(n);
