@@ -98,7 +98,10 @@ def test_permute(ctx, dtype, shape, perm):
 @pytest.mark.parametrize("sa,sb,aa,ab", [((4, 5, 6), (6, 7), [2], [0]), ((4, 5, 6), (7, 4), [0], [1]),
                                          ((3, 4, 5, 6), (5, 3, 8), [2, 0], [0, 1]), ((30, 40), (40, 50), [1], [0]),
                                          ((3, 4), (5, 6), [], []), ((9, 70, 11), (11, 70, 2), [1, 2], [1, 0]),
-                                         ((130, 2, 129), (129, 2, 131), [2], [0])])
+                                         ((130, 2, 129), (129, 2, 131), [2], [0]),
+                                         # few output tiles and a long K: split over K (real case), the
+                                         # last split shorter than the others
+                                         ((64, 2000), (2000, 48), [1], [0]), ((33, 1037), (70, 1037), [1], [1])])
 @pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
 def test_tensordot(ctx, dtype, sa, sb, aa, ab, conj):
     import networksolvers_amd as ns
