@@ -14,6 +14,7 @@
 // Every offset is checked against its buffer's size when the program is created (x
 // and y when it runs), so a kernel never sees an out-of-range block.
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <numeric>
 
@@ -207,15 +208,14 @@ struct BsProgram {
   std::vector<BsStep> steps;
   int64_t x_elems = 0, y_elems = 0;
   double flops = 0.0;
+  // every step's item, segment / term and schedule tables, in one device block
+  // uploaded once (the steps point into it)
+  void* tables = nullptr;
 };
 
 void bs_release(nsmi_ctx* ctx, BsProgram* p) {
   if (!p) return;
-  for (auto& s : p->steps) {
-    scratch_release(ctx, s.items);
-    scratch_release(ctx, s.aux);
-    scratch_release(ctx, s.sched);
-  }
+  scratch_release(ctx, p->tables);
   for (void* s : p->scratch) scratch_release(ctx, s);
   for (void* s : p->op32) scratch_release(ctx, s);
   scratch_release(ctx, p->xlo);
@@ -225,13 +225,24 @@ void bs_release(nsmi_ctx* ctx, BsProgram* p) {
 
 namespace {
 
-template <typename T>
-void* upload_vec(nsmi_ctx* ctx, const std::vector<T>& v) {
-  if (v.empty()) return nullptr;
-  void* d = scratch(ctx, sizeof(T) * v.size());
-  NSMI_HIP(hipMemcpyAsync(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
-  return d;
-}
+// Host image of a program's tables: each add() appends one table at a 16-byte
+// aligned offset and returns that offset + 1 (0 for an empty table), which the
+// steps keep in their pointer fields until the one upload rebases them.
+struct HostTables {
+  std::vector<unsigned char> bytes;
+  template <typename T>
+  void* add(const std::vector<T>& v) {
+    if (v.empty()) return nullptr;
+    const size_t off = (bytes.size() + 15) & ~size_t(15);
+    bytes.resize(off + sizeof(T) * v.size());
+    std::memcpy(bytes.data() + off, v.data(), sizeof(T) * v.size());
+    return reinterpret_cast<void*>(off + 1);
+  }
+  static void* rebase(void* tag, void* base) {
+    if (!tag) return nullptr;
+    return static_cast<unsigned char*>(base) + (reinterpret_cast<uintptr_t>(tag) - 1);
+  }
+};
 
 void check_range(const BsProgram* p, int buf, int64_t off, int64_t rows, int64_t ld, int64_t cols,
                  const char* what) {
@@ -510,6 +521,7 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
     p->xlo = scratch(ctx, sizeof(float) * size_t(std::max<int64_t>(x_elems, 1)));
     p->ylo = scratch(ctx, sizeof(float) * size_t(std::max<int64_t>(y_elems, 1)));
   }
+  HostTables ht;
   for (int si = 0; si < n_steps; ++si) {
     const int64_t* st = steps + 6 * si;  // kind, transA, transB, first, count, aux
     BsStep s;
@@ -577,9 +589,9 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
       s.nsched64 = int64_t(tl.size()) - s.nsched;
       if (!wide) s.tile = 32;
       require(tl.size() < (size_t(1) << 31), "bsop: too many tiles");
-      s.items = upload_vec(ctx, its);
-      s.aux = upload_vec(ctx, sg);
-      s.sched = upload_vec(ctx, tl);
+      s.items = ht.add(its);
+      s.aux = ht.add(sg);
+      s.sched = ht.add(tl);
       s.tune_key = tune_key(s, its, sg, all, big);
     } else if (s.kind == 1) {
       std::vector<LItem> its;
@@ -606,9 +618,9 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
         for (int64_t c0 = 0; c0 < it.n; c0 += 256) ch.push_back(Chunk{int(k), 0, c0});
         its.push_back(it);
       }
-      s.items = upload_vec(ctx, its);
-      s.aux = upload_vec(ctx, tm);
-      s.sched = upload_vec(ctx, ch);
+      s.items = ht.add(its);
+      s.aux = ht.add(tm);
+      s.sched = ht.add(ch);
       s.nsched = int64_t(ch.size());
     } else if (s.kind == 2) {
       std::vector<CItem> its;
@@ -621,8 +633,8 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
         for (int64_t c0 = 0; c0 < it.rows * it.cols; c0 += 256) ch.push_back(Chunk{int(k), 0, c0});
         its.push_back(it);
       }
-      s.items = upload_vec(ctx, its);
-      s.sched = upload_vec(ctx, ch);
+      s.items = ht.add(its);
+      s.sched = ht.add(ch);
       s.nsched = int64_t(ch.size());
     } else {
       fail(NSMI_ERR_INVALID, "bsop: unknown step kind");
@@ -630,6 +642,15 @@ int nsmi_bsop_create(nsmi_ctx* ctx, int n_operands, const nsmi_tensor* const* op
     require(s.nsched < (int64_t(1) << 31), "bsop: step too large for one launch");
     p->flops += s.flops;
     p->steps.push_back(s);
+  }
+  if (!ht.bytes.empty()) {
+    p->tables = scratch(ctx, ht.bytes.size());
+    NSMI_HIP(hipMemcpyAsync(p->tables, ht.bytes.data(), ht.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+  }
+  for (BsStep& s : p->steps) {
+    s.items = HostTables::rebase(s.items, p->tables);
+    s.aux = HostTables::rebase(s.aux, p->tables);
+    s.sched = HostTables::rebase(s.sched, p->tables);
   }
   *out = heff_wrap_bs(p, x_elems == y_elems ? x_elems : -1);
   p = nullptr;

@@ -109,6 +109,34 @@ void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t) {
 }
 
 void* scratch(nsmi_ctx* ctx, size_t bytes) { return ctx->pool->alloc(bytes); }
+
+void* stage_take(nsmi_ctx* ctx, size_t bytes) {
+  bytes = std::max<size_t>((bytes + 15) & ~size_t(15), 16);
+  auto& ch = ctx->h_stage;
+  while (ctx->h_stage_cur < ch.size() && ctx->h_stage_off + bytes > ch[ctx->h_stage_cur].second) {
+    ++ctx->h_stage_cur;
+    ctx->h_stage_off = 0;
+  }
+  if (ctx->h_stage_cur == ch.size()) {
+    const size_t sz = std::max<size_t>(bytes, size_t(1) << 20);
+    void* p = nullptr;
+    NSMI_HIP(hipHostMalloc(&p, sz, hipHostMallocDefault));
+    ch.emplace_back(p, sz);
+    ctx->h_stage_off = 0;
+  }
+  void* p = static_cast<unsigned char*>(ch[ctx->h_stage_cur].first) + ctx->h_stage_off;
+  ctx->h_stage_off += bytes;
+  return p;
+}
+
+void stage_reset(nsmi_ctx* ctx) {
+  if (ctx->h_stage_busy) NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->h_stage_busy = true;
+  ctx->h_stage_cur = 0;
+  ctx->h_stage_off = 0;
+}
+
+void stage_done(nsmi_ctx* ctx) { ctx->h_stage_busy = false; }
 void scratch_release(nsmi_ctx* ctx, void* p) {
   if (ctx->defer_release)
     ctx->deferred.push_back(p);
@@ -310,6 +338,7 @@ int nsmi_ctx_destroy(nsmi_ctx* ctx) {
   if (ctx->d_partials) (void)hipFree(ctx->d_partials);
   if (ctx->d_scal) (void)hipFree(ctx->d_scal);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  for (auto& c : ctx->h_stage) (void)hipHostFree(c.first);
   if (ctx->d_info) (void)hipFree(ctx->d_info);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;

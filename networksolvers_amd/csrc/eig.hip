@@ -759,8 +759,9 @@ bool sytrd_launch(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double*
       NSMI_HIP(hipLaunchKernel(p.fn, dim3(unsigned(G)), dim3(kTriThreads), args, 0, ctx->stream));
     }
   }
+  run.host_flags = static_cast<unsigned*>(stage_take(ctx, 2 * sizeof(unsigned)));
   run.host_flags[0] = run.host_flags[1] = 0u;
-  NSMI_HIP(hipMemcpyAsync(run.host_flags, failp, sizeof(run.host_flags), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipMemcpyAsync(run.host_flags, failp, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   return true;
 }
 

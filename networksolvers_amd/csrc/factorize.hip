@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <memory>
 #include <numeric>
@@ -227,8 +228,8 @@ struct FactCore {
   std::vector<int> order;
   std::unique_ptr<Scratch> rho, z, tau, wdev, w, e;
   SytrdRun sr;             // the tridiagonal reduction's buffers and host flags
-  std::vector<double> wh;  // host copy of the eigenvalues (filled asynchronously)
-  int info_h = 0;          // host copy of the solver status
+  double* wh = nullptr;    // host copy of the eigenvalues (pinned stage memory, filled asynchronously)
+  int* info_h = nullptr;   // host copy of the solver status (pinned stage memory)
   FactCore(const double* M_, int64_t m_, int64_t nc_, bool rt, int r) : M(M_), m(m_), nc(nc_), rows_trailing(rt),
                                                                          route(r) {
     iso = rt ? nc : m;
@@ -294,10 +295,11 @@ bool fact_tri_launch(nsmi_ctx* ctx, FactCore& f) {
 }
 
 void fact_readback(nsmi_ctx* ctx, FactCore& f) {
-  f.wh.assign(size_t(f.iso), 0.0);
-  f.info_h = 0;
-  NSMI_HIP(hipMemcpyAsync(&f.info_h, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  NSMI_HIP(hipMemcpyAsync(f.wh.data(), f.w->p, sizeof(double) * f.iso, hipMemcpyDeviceToHost, ctx->stream));
+  f.wh = static_cast<double*>(stage_take(ctx, sizeof(double) * size_t(f.iso)));
+  f.info_h = static_cast<int*>(stage_take(ctx, sizeof(int)));
+  *f.info_h = 0;
+  NSMI_HIP(hipMemcpyAsync(f.info_h, ctx->d_info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  NSMI_HIP(hipMemcpyAsync(f.wh, f.w->p, sizeof(double) * f.iso, hipMemcpyDeviceToHost, ctx->stream));
 }
 
 // after the reduction's flags are on the host
@@ -331,10 +333,11 @@ void fact_syevd(nsmi_ctx* ctx, FactCore& f) {
 
 // after the stream has passed fact_readback's copies
 void fact_finish(FactCore& f) {
-  if (f.info_h != 0)
+  if (*f.info_h != 0)
     fail(NSMI_ERR_SOLVER, std::string(f.tri ? "rocsolver_dstedc" : "rocsolver_dsyevd") + " returned info = " +
-                              std::to_string(f.info_h));
-  fact_spectrum(f, f.wh);
+                              std::to_string(*f.info_h));
+  std::vector<double> w(f.wh, f.wh + f.iso);
+  fact_spectrum(f, w);
 }
 
 // one factorisation on the main stream
@@ -364,9 +367,8 @@ void fact_jacobi_launch(nsmi_ctx* ctx, const std::vector<FactCore*>& fs, unsigne
   jacobi_eig_batched(ctx, jobs, unconverged);
   for (FactCore* f : fs)
     if (f->jacobi) {
-      f->wh.assign(size_t(f->iso), 0.0);
-      NSMI_HIP(hipMemcpyAsync(f->wh.data(), f->wdev->p, sizeof(double) * f->iso, hipMemcpyDeviceToHost,
-                              ctx->stream));
+      f->wh = static_cast<double*>(stage_take(ctx, sizeof(double) * size_t(f->iso)));
+      NSMI_HIP(hipMemcpyAsync(f->wh, f->wdev->p, sizeof(double) * f->iso, hipMemcpyDeviceToHost, ctx->stream));
     }
 }
 
@@ -374,16 +376,17 @@ void fact_jacobi_finish(const std::vector<FactCore*>& fs, unsigned unconverged) 
   if (unconverged) fail(NSMI_ERR_SOLVER, "jacobi: a Gram matrix did not converge within the sweep limit");
   for (FactCore* f : fs)
     if (f->jacobi) {
-      fact_spectrum(*f, f->wh);
+      std::vector<double> w(f->wh, f->wh + f->iso);
+      fact_spectrum(*f, w);
       f->jacobi = false;
     }
 }
 
 void fact_jacobi(nsmi_ctx* ctx, const std::vector<FactCore*>& fs) {
-  unsigned unconverged = 0;
-  fact_jacobi_launch(ctx, fs, &unconverged);
+  unsigned* unconverged = static_cast<unsigned*>(stage_take(ctx, sizeof(unsigned)));
+  fact_jacobi_launch(ctx, fs, unconverged);
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
-  fact_jacobi_finish(fs, unconverged);
+  fact_jacobi_finish(fs, *unconverged);
 }
 
 void fact_prepare(nsmi_ctx* ctx, FactCore& f) {
@@ -404,7 +407,9 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
   const int* vidx = d_idx.as<int>();
   std::unique_ptr<Scratch> zk;
   {
-    NSMI_HIP(hipMemcpyAsync(d_idx.p, f.order.data(), sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
+    int* hidx = static_cast<int*>(stage_take(ctx, sizeof(int) * size_t(k)));
+    std::memcpy(hidx, f.order.data(), sizeof(int) * size_t(k));
+    NSMI_HIP(hipMemcpyAsync(d_idx.p, hidx, sizeof(int) * k, hipMemcpyHostToDevice, ctx->stream));
     if (f.tri) {
       // kept eigenvectors of T (columns order[0..k) of z), then Q . Zk with the reflectors
       ProfScope ps2(ctx, NSMI_K_FACT_SOLVE, 2.0 * double(iso) * iso * (2.0 * k + iso), 0.0);
@@ -437,8 +442,8 @@ void fact_emit(nsmi_ctx* ctx, FactCore& f, int64_t k, double* U, double* C) {
       NSMI_HIP(hipMemcpyAsync(U, Ut.p, sizeof(double) * k * nc, hipMemcpyDeviceToDevice, ctx->stream));
       gemm_f64(ctx, gdesc(m, k, nc, false, true), 1.0, f.M, Ut.as<double>(), 0.0, C, kRoleFact);
     }
-    // f.order (the source of the index copy above) lives as long as f, which the
-    // entry points keep until their closing stream synchronisation
+    // the index copy above reads pinned stage memory, which stays untouched until the
+    // next entry point resets the stage after this call's closing synchronisation
   }
 }
 
@@ -464,6 +469,7 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
                    const nsmi_trunc* trunc, nsmi_tensor** Uout, nsmi_tensor** Cout, nsmi_trunc_info* info) {
   if (!ctx || !theta || !trunc || !Uout || !Cout) return NSMI_ERR_INVALID;
   API_BEGIN
+  stage_reset(ctx);
   *Uout = nullptr;
   *Cout = nullptr;
   require(theta->dtype == NSMI_F64 || theta->dtype == NSMI_C128, "factorize: unknown element type");
@@ -535,6 +541,7 @@ int nsmi_factorize(nsmi_ctx* ctx, const nsmi_tensor* theta, int n_row_dims, int 
   // scratch blocks are returned to the pool in stream order; the host vectors
   // copied above were synchronised already
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  stage_done(ctx);
   API_END(ctx)
 }
 
@@ -647,6 +654,7 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
     return NSMI_ERR_INVALID;
   std::vector<nsmi_tensor*> made;
   API_BEGIN
+  stage_reset(ctx);
   *Uall = nullptr;
   *Call = nullptr;
   require(Mall->dtype == NSMI_F64, "factorize_sectors: sector matrices are FP64");
@@ -698,14 +706,14 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
     on_stream(t, [&]() {
       if (!fact_tri_launch(ctx, *big[t])) lib.push_back(big[t]);
     });
-  unsigned unconverged = 0;
-  fact_jacobi_launch(ctx, fcp, &unconverged);
+  unsigned* unconverged = static_cast<unsigned*>(stage_take(ctx, sizeof(unsigned)));
+  fact_jacobi_launch(ctx, fcp, unconverged);
   sync_all();
   for (size_t t = 0; t < big.size(); ++t)
     if (big[t]->tri) on_stream(t, [&]() { fact_tri_solve(ctx, *big[t]); });
   for (FactCore* f : lib) fact_syevd(ctx, *f);
   sync_all();
-  fact_jacobi_finish(fcp, unconverged);
+  fact_jacobi_finish(fcp, *unconverged);
   for (FactCore* f : big) fact_finish(*f);
   auto key = [&](double v) { return route == NSMI_ROUTE_EIGEN ? std::fabs(v) : v; };
   struct Entry {
@@ -769,6 +777,7 @@ int nsmi_factorize_sectors(nsmi_ctx* ctx, const nsmi_tensor* Mall, int n, const 
     info->route = route;
   }
   NSMI_HIP(hipStreamSynchronize(ctx->stream));
+  stage_done(ctx);
   *Uall = Ut;
   *Call = Ct;
   made.clear();

@@ -88,6 +88,11 @@ struct nsmi_ctx {
   double* d_partials = nullptr;  // kPartials doubles
   double* d_scal = nullptr;      // 64 result slots
   double* h_scal = nullptr;      // pinned mirror of d_scal
+  // pinned host chunks for small asynchronous read-backs and uploads of one call
+  // (stage_take / stage_reset); never freed before the context
+  std::vector<std::pair<void*, size_t>> h_stage;
+  size_t h_stage_cur = 0, h_stage_off = 0;
+  bool h_stage_busy = false;  // a call took stage memory and has not reached stage_done
   int* d_info = nullptr;
   // auxiliary streams (default priority), each with its own rocBLAS handle and
   // solver status word, for independent sector solves; created on first use
@@ -130,6 +135,16 @@ inline size_t dtype_size(int dt) {
 nsmi_tensor* tensor_new(nsmi_ctx* ctx, int dtype, int rank, const int64_t* dims);
 void tensor_delete(nsmi_ctx* ctx, nsmi_tensor* t);
 void* scratch(nsmi_ctx* ctx, size_t bytes);        // pool block, caller releases
+// Pinned host memory for a call's small device<->host copies, which then stay
+// asynchronous (a copy into pageable memory blocks the host until the stream gets
+// there, which would serialise work spread over several streams).  stage_take
+// hands out 16-byte aligned slices; stage_reset, at the start of an entry point,
+// makes the whole arena reusable, and stage_done, after that entry point's closing
+// synchronisation, marks it idle.  A reset after a call that ended early (an error)
+// first waits for the main stream, so no copy of that call is still in flight.
+void* stage_take(nsmi_ctx* ctx, size_t bytes);
+void stage_reset(nsmi_ctx* ctx);
+void stage_done(nsmi_ctx* ctx);
 void scratch_release(nsmi_ctx* ctx, void* p);
 
 // Work on the auxiliary streams: AuxSection creates them (k at most), holds pool
@@ -316,7 +331,7 @@ int sytrd_coop_groups(int n);
 // host copy of the two failure flags (barrier time-out, non-finite d or e)
 struct SytrdRun {
   std::unique_ptr<Scratch> xch, flags;
-  unsigned host_flags[2] = {0u, 0u};
+  unsigned* host_flags = nullptr;  // two flags in pinned stage memory
 };
 bool sytrd_launch(nsmi_ctx* ctx, double* A, int n, double* d, double* e, double* tau, SytrdRun& run);
 void sytrd_check(const SytrdRun& run);
