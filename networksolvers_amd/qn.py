@@ -381,26 +381,55 @@ class _Mix:
         return sY
 
 
+def _w_nonzeros(W, out_axis):
+    """Nonzero entries of the MPO tensor W[wl, wr, s', s] grouped by (the other MPO
+    index, s): lists of (out index, s', c) in the order of the loops over the output
+    MPO index, then s'."""
+    nz = {}
+    Dl, Dr, dp, d = W.shape
+    for i in range(Dl if out_axis == 1 else Dr):
+        for s in range(d):
+            lst = []
+            for o in range(Dr if out_axis == 1 else Dl):
+                for sp in range(dp):
+                    c = float(W[i, o, sp, s] if out_axis == 1 else W[o, i, sp, s])
+                    if c != 0.0:
+                        lst.append((o, sp, c))
+            nz[(i, s)] = lst
+    return nz
+
+
 def _mix_left(E: QNTensor, A: QNTensor, W, cl) -> _Mix:
     """Y[(ap, sp, wp, b)] = sum_{w, s} W[w, wp, sp, s] (E[ap, w, a] A[a, s, b]), blocks
     l.n[ap] x r.n[b], for A[l, s, r] and the environment E[l', w, l] on its left."""
     l, site, r = A.indices
+    # per left sector a: the environment blocks E[a', w, a] that exist, in w order
+    left = []
+    for a in range(len(l)):
+        lst = []
+        for w, cw in enumerate(cl):
+            ap = l.find(l.q[a] + cw)
+            i = None if ap is None else E.pos.get((ap, w, a))
+            if i is not None:
+                lst.append((w, ap, E.offs[i]))
+        left.append(lst)
     X, X_off, xs = {}, 0, []
-    for (a, s, b) in A.keys:
-        for w in range(len(cl)):
-            ap = l.find(l.q[a] + cl[w])
-            if ap is None or not E.has((ap, w, a)):
-                continue
+    for (a, s, b), ao in zip(A.keys, A.offs):
+        for w, ap, eo in left[a]:
             X[(ap, w, s, b)] = X_off
-            xs.append(((ap, w, s, b), E.block_off((ap, w, a)), A.block_off((a, s, b)), l.n[ap], l.n[a], r.n[b]))
+            xs.append(((ap, w, s, b), eo, ao, l.n[ap], l.n[a], r.n[b]))
             X_off += l.n[ap] * r.n[b]
+    wnz = _w_nonzeros(W, 1)
     ys = {}
-    for (ap, w, s, b) in X:
-        for wp in range(W.shape[1]):
-            for sp in range(W.shape[2]):
-                c = W[w, wp, sp, s]
-                if c != 0.0:
-                    ys.setdefault((ap, sp, wp, b), []).append(((ap, w, s, b), c))
+    for k in X:
+        ap, w, s, b = k
+        for wp, sp, c in wnz[(w, s)]:
+            kk = (ap, sp, wp, b)
+            lst = ys.get(kk)
+            if lst is None:
+                ys[kk] = [(k, c)]
+            else:
+                lst.append((k, c))
     return _Mix(True, xs, X, X_off, ys, {k: l.n[k[0]] * r.n[k[3]] for k in ys})
 
 
@@ -408,22 +437,33 @@ def _mix_right(E: QNTensor, A: QNTensor, W, cr) -> _Mix:
     """Y[(a, w, sp, bp)] = sum_{v, s} W[w, v, sp, s] (A[a, s, b] E[bp, v, b]), blocks
     l.n[a] x r.n[bp], for A[l, s, r] and the environment E[r', v, r] on its right."""
     l, site, r = A.indices
+    # per right sector b: the environment blocks E[b', v, b] that exist, in v order
+    right = []
+    for b in range(len(r)):
+        lst = []
+        for v, cv in enumerate(cr):
+            bp = r.find(r.q[b] + cv)
+            i = None if bp is None else E.pos.get((bp, v, b))
+            if i is not None:
+                lst.append((v, bp, E.offs[i]))
+        right.append(lst)
     X, X_off, xs = {}, 0, []
-    for (a, s, b) in A.keys:
-        for v in range(len(cr)):
-            bp = r.find(r.q[b] + cr[v])
-            if bp is None or not E.has((bp, v, b)):
-                continue
+    for (a, s, b), ao in zip(A.keys, A.offs):
+        for v, bp, eo in right[b]:
             X[(a, s, v, bp)] = X_off
-            xs.append(((a, s, v, bp), A.block_off((a, s, b)), E.block_off((bp, v, b)), l.n[a], r.n[b], r.n[bp]))
+            xs.append(((a, s, v, bp), ao, eo, l.n[a], r.n[b], r.n[bp]))
             X_off += l.n[a] * r.n[bp]
+    wnz = _w_nonzeros(W, 0)
     ys = {}
-    for (a, s, v, bp) in X:
-        for w in range(W.shape[0]):
-            for sp in range(W.shape[2]):
-                c = W[w, v, sp, s]
-                if c != 0.0:
-                    ys.setdefault((a, w, sp, bp), []).append(((a, s, v, bp), c))
+    for k in X:
+        a, s, v, bp = k
+        for w, sp, c in wnz[(v, s)]:
+            kk = (a, w, sp, bp)
+            lst = ys.get(kk)
+            if lst is None:
+                ys[kk] = [(k, c)]
+            else:
+                lst.append((k, c))
     return _Mix(False, xs, X, X_off, ys, {k: l.n[k[0]] * r.n[k[3]] for k in ys})
 
 
@@ -439,14 +479,17 @@ def env_left(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
     xE, xA = p.operand(E.data), 0
     sY = mix.emit(p, xE, xA)
     Y = mix.Y
+    by_last = {}   # A's blocks grouped by their right sector, in A's key order
+    for (ap, sp, bp), ao in zip(A.keys, A.offs):
+        by_last.setdefault(bp, []).append((ap, sp, ao))
     items = []
-    for (bp, wp, b) in out.keys:
+    for (bp, wp, b), oo in zip(out.keys, out.offs):
         sg = []
-        for (ap, sp, bpp) in A.keys:
-            if bpp != bp or (ap, sp, wp, b) not in Y:
-                continue
-            sg.append((xA, A.block_off((ap, sp, bp)), r.n[bp], sY, Y[(ap, sp, wp, b)], r.n[b], l.n[ap], 1.0))
-        items.append((1, out.block_off((bp, wp, b)), r.n[b], r.n[bp], r.n[b], sg))
+        for ap, sp, ao in by_last.get(bp, ()):
+            y = Y.get((ap, sp, wp, b))
+            if y is not None:
+                sg.append((xA, ao, r.n[bp], sY, y, r.n[b], l.n[ap], 1.0))
+        items.append((1, oo, r.n[b], r.n[bp], r.n[b], sg))
     p.gemm(True, False, items)
     _run_once(p, ctx, A, out)
     return out
@@ -463,14 +506,17 @@ def env_right(E: QNTensor, A: QNTensor, W, cl, cr, ctx) -> QNTensor:
     xE, xA = p.operand(E.data), 0
     sY = mix.emit(p, xE, xA)
     Y = mix.Y
+    by_first = {}   # A's blocks grouped by their left sector, in A's key order
+    for (app, sp, bp), ao in zip(A.keys, A.offs):
+        by_first.setdefault(app, []).append((sp, bp, ao))
     items = []
-    for (ap, w, a) in out.keys:
+    for (ap, w, a), oo in zip(out.keys, out.offs):
         sg = []
-        for (app, sp, bp) in A.keys:
-            if app != ap or (a, w, sp, bp) not in Y:
-                continue
-            sg.append((xA, A.block_off((ap, sp, bp)), r.n[bp], sY, Y[(a, w, sp, bp)], r.n[bp], r.n[bp], 1.0))
-        items.append((1, out.block_off((ap, w, a)), l.n[a], l.n[ap], l.n[a], sg))
+        for sp, bp, ao in by_first.get(ap, ()):
+            y = Y.get((a, w, sp, bp))
+            if y is not None:
+                sg.append((xA, ao, r.n[bp], sY, y, r.n[bp], r.n[bp], 1.0))
+        items.append((1, oo, l.n[a], l.n[ap], l.n[a], sg))
     p.gemm(False, True, items)
     _run_once(p, ctx, A, out)
     return out
@@ -497,50 +543,91 @@ def theta(A: QNTensor, B: QNTensor, ctx) -> QNTensor:
     return th
 
 
+_W12_NZ = {}
+
+
+def _w12_nonzeros(W1, W2):
+    """Nonzero entries of W12[w, s1, s2, s1', s2', v] = sum_m W1[w, m, s1', s1] W2[m, v, s2', s2],
+    grouped by (w, s1, s2) in row-major order; cached per pair of MPO tensors."""
+    key = (W1.shape, W2.shape, W1.tobytes(), W2.tobytes())
+    nz = _W12_NZ.get(key)
+    if nz is None:
+        W12 = np.einsum("wmas,mvbt->wstabv", W1, W2)   # [w, s1, s2, s1', s2', v]
+        nz = {}
+        for w, s1, s2, a1, a2, v in zip(*np.nonzero(W12)):
+            nz.setdefault((int(w), int(s1), int(s2)), []).append((int(a1), int(a2), int(v),
+                                                                  float(W12[w, s1, s2, a1, a2, v])))
+        if len(_W12_NZ) > 256:
+            _W12_NZ.clear()
+        _W12_NZ[key] = nz
+    return nz
+
+
 def heff2(L: QNTensor, W1, W2, R: QNTensor, th: QNTensor, cL, cR, ctx, precision="fp64") -> BlockProgram:
     """Two-site effective Hamiltonian (operator_map.jl:3-10) on theta's block space:
     T1 = L.theta (GEMM), T3 = (W1 W2)-mix (LINCOMB), y = T3.R^T (GEMM, segments over
     the right MPO link); in FP64 or the FP32 form."""
     l, s1i, s2i, r = th.indices
-    W12 = np.einsum("wmas,mvbt->wstabv", W1, W2)   # [w, s1, s2, s1', s2', v]
-    nz = {}
-    for w, s1, s2, a1, a2, v in zip(*np.nonzero(W12)):
-        nz.setdefault((w, s1, s2), []).append((a1, a2, v, W12[w, s1, s2, a1, a2, v]))
+    nz = _w12_nonzeros(W1, W2)
     p = _Program()
     xL, xR = p.operand(L.data), p.operand(R.data)
+    ln, rn = l.n, r.n
+    # per left sector a: the environment blocks L[a', w, a] that exist, in w order
+    left = []
+    for a in range(len(l)):
+        lst = []
+        for w, cw in enumerate(cL):
+            ap = l.find(l.q[a] + cw)
+            i = None if ap is None else L.pos.get((ap, w, a))
+            if i is not None:
+                lst.append((w, ap, L.offs[i]))
+        left.append(lst)
     T1, T1_off, t1 = {}, 0, []
-    for (a, s1, s2, b) in th.keys:
-        for w in range(len(cL)):
-            ap = l.find(l.q[a] + cL[w])
-            if ap is None or not L.has((ap, w, a)):
-                continue
-            T1[(ap, w, s1, s2, b)] = T1_off
-            t1.append(((ap, w, s1, s2, b), L.block_off((ap, w, a)), th.block_off((a, s1, s2, b)), l.n[ap], l.n[a],
-                       r.n[b]))
-            T1_off += l.n[ap] * r.n[b]
+    for (a, s1, s2, b), to in zip(th.keys, th.offs):
+        nb = rn[b]
+        for w, ap, lo in left[a]:
+            k = (ap, w, s1, s2, b)
+            T1[k] = T1_off
+            t1.append((k, lo, to, ln[ap], ln[a], nb))
+            T1_off += ln[ap] * nb
     t3 = {}
-    for (ap, w, s1, s2, b) in T1:
-        for a1, a2, v, c in nz.get((w, s1, s2), []):
-            t3.setdefault((ap, a1, a2, v, b), []).append(((ap, w, s1, s2, b), c))
+    for k in T1:
+        ap, w, s1, s2, b = k
+        for a1, a2, v, c in nz.get((w, s1, s2), ()):
+            kk = (ap, a1, a2, v, b)
+            lst = t3.get(kk)
+            if lst is None:
+                t3[kk] = [(k, c)]
+            else:
+                lst.append((k, c))
     T3, T3_off = {}, 0
     for k in t3:
         T3[k] = T3_off
-        T3_off += l.n[k[0]] * r.n[k[4]]
+        T3_off += ln[k[0]] * rn[k[4]]
     p.scratch_buf(T1_off)
     p.scratch_buf(T3_off)
     s1b, s3b = p.sid(0), p.sid(1)
     p.gemm(False, False, [(s1b, T1[k], nb, nap, nb, [(xL, lo, na, 0, to, nb, na, 1.0)])
                           for (k, lo, to, nap, na, nb) in t1])
-    p.lincomb([(s3b, T3[k], l.n[k[0]] * r.n[k[4]], [(s1b, T1[tk], c) for tk, c in tl]) for k, tl in t3.items()])
+    p.lincomb([(s3b, T3[k], ln[k[0]] * rn[k[4]], [(s1b, T1[tk], c) for tk, c in tl]) for k, tl in t3.items()])
+    # per right sector b': the environment blocks R[b', v, b] that exist, in v order
+    right = []
+    for bp in range(len(r)):
+        lst = []
+        for v, cv in enumerate(cR):
+            b = r.find(r.q[bp] - cv)
+            i = None if b is None else R.pos.get((bp, v, b))
+            if i is not None:
+                lst.append((v, b, R.offs[i]))
+        right.append(lst)
     items = []
-    for (ap, a1, a2, bp) in th.keys:
+    for (ap, a1, a2, bp), to in zip(th.keys, th.offs):
         sg = []
-        for v in range(len(cR)):
-            b = r.find(r.q[bp] - cR[v])
-            if b is None or (ap, a1, a2, v, b) not in T3 or not R.has((bp, v, b)):
-                continue
-            sg.append((s3b, T3[(ap, a1, a2, v, b)], r.n[b], xR, R.block_off((bp, v, b)), r.n[b], r.n[b], 1.0))
-        items.append((1, th.block_off((ap, a1, a2, bp)), r.n[bp], l.n[ap], r.n[bp], sg))
+        for v, b, ro in right[bp]:
+            t = T3.get((ap, a1, a2, v, b))
+            if t is not None:
+                sg.append((s3b, t, rn[b], xR, ro, rn[b], rn[b], 1.0))
+        items.append((1, to, rn[bp], ln[ap], rn[bp], sg))
     p.gemm(False, True, items)
     return p.build(ctx, th.size, th.size, precision)
 
