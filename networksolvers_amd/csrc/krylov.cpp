@@ -69,6 +69,13 @@ struct KOps {
       vec_dot_dev(ctx, a, b, n, slot);
   }
   void norm2_dev(const double* a, int slot) const { vec_dot_dev(ctx, a, a, is_c ? 2 * n : n, slot); }
+  // r += f <q, r> q, the dot product also left in slot: one Gram-Schmidt step
+  void dot_axpy(const double* q, double* r, int slot, double f) const {
+    if (is_c)
+      vec_zdot_axpy(ctx, q, r, n, slot, f);
+    else
+      vec_dot_axpy(ctx, q, r, n, slot, f);
+  }
   // y += f * c x with c the (complex) slot value, or y += f * (Re c / div) x when div != 0
   void axpy_dc(double* y, const double* x, int slot, double f, double div = 0.0) const {
     if (is_c)
@@ -107,8 +114,7 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
   ops.scaled_copy(vk, r, 1.0 / beta);
   heff_apply_dev(ops.ctx, op, vk, r);
   ops.axpy(r, vprev, cplx(-beta, 0.0));
-  ops.dot_dev(vk, r, kSlotDot);
-  ops.axpy_dc(r, vk, kSlotDot, -1.0);
+  ops.dot_axpy(vk, r, kSlotDot, -1.0);
   ops.norm2_dev(r, kSlotN2a);
   read_slots(ops.ctx, 3, sl);
   double a = sl[0];
@@ -117,9 +123,7 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
   while (kEps < b && b < eta * nold) {
     nold = b;
     for (int j = 0; j <= k; ++j) {
-      const double* q = V + nd * j;
-      ops.dot_dev(q, r, kSlotDot);
-      ops.axpy_dc(r, q, kSlotDot, -1.0);
+      ops.dot_axpy(V + nd * j, r, kSlotDot, -1.0);
     }
     ops.norm2_dev(r, kSlotN2a);
     read_slots(ops.ctx, 3, sl);
@@ -148,8 +152,7 @@ void lanczos_start(const KOps& ops, nsmi_heff* op, const double* x0, double beta
   double beta = std::sqrt(std::max(sl[kSlotN2b], 0.0));
   while (kEps < beta && beta < eta * beta_old) {
     beta_old = beta;
-    ops.dot_dev(V, r, kSlotDot);
-    ops.axpy_dc(r, V, kSlotDot, -1.0);
+    ops.dot_axpy(V, r, kSlotDot, -1.0);
     ops.norm2_dev(r, kSlotN2b);
     read_slots(ops.ctx, 4, sl);
     alpha += sl[0];

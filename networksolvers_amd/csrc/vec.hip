@@ -147,6 +147,29 @@ __global__ void k_nonfinite_partial(const double* __restrict__ x, int64_t n, dou
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// The finishing pass of a dot product folded into the axpy that consumes it: every
+// block adds the partials in the order k_finish uses, so all blocks -- and the slot
+// block 0 leaves for the host -- see the same coefficient c; then y += f c x.
+// (The partials come from the previous launch on the stream, so no fence is needed.)
+__device__ inline double partials_sum(const double* __restrict__ partials, int np) {
+  __shared__ double total;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += partials[i];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) total = s;
+  __syncthreads();
+  return total;
+}
+
+__global__ void k_axpy_dp(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                          const double* __restrict__ partials, int np, double f, double* __restrict__ slot) {
+  const double c = partials_sum(partials, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) slot[0] = c;
+  const double a = f * c;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    y[i] += a * x[i];
+}
+
 // ---- complex (interleaved re, im) vectors; n counts complex elements --------
 __global__ void k_zscale(double* __restrict__ x, int64_t n, double ar, double ai) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
@@ -190,6 +213,22 @@ __global__ void k_zdot_partial(const double* __restrict__ x, const double* __res
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = sr;
     partials[kPartials + blockIdx.x] = si;
+  }
+}
+
+__global__ void k_zaxpy_dp(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                           const double* __restrict__ partials, int np, double f, double* __restrict__ slot) {
+  const double cr = partials_sum(partials, np);
+  const double ci = partials_sum(partials + kPartials, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    slot[0] = cr;
+    slot[1] = ci;
+  }
+  const double ar = f * cr, ai = f * ci;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double xr = x[2 * i], xi = x[2 * i + 1];
+    y[2 * i] += ar * xr - ai * xi;
+    y[2 * i + 1] += ar * xi + ai * xr;
   }
 }
 
@@ -284,6 +323,28 @@ void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 48.0 * n);
   hipLaunchKernelGGL(k_zaxpy_dc, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, y, x, n, ctx->d_scal + slot, f,
                      div);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f) {
+  require(slot >= 0 && slot < 64, "slot out of range");
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 40.0 * n);
+  const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_axpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
+                     ctx->d_scal + slot);
+  NSMI_LAUNCH_CHECK();
+}
+
+void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f) {
+  require(slot >= 0 && slot + 1 < 64, "slot out of range");
+  ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 80.0 * n);
+  const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
+  hipLaunchKernelGGL(k_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_zaxpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
+                     ctx->d_scal + slot);
   NSMI_LAUNCH_CHECK();
 }
 

@@ -64,6 +64,9 @@ def spin_site_index(d):
 
 
 # ------------------------------------------------------------------ tensors
+_KEYS_CACHE = {}
+
+
 class QNTensor:
     """A block-sparse tensor: ``keys`` (tuples of sector ids) in storage order, each
     block dense with the sector dims, all in one flat device buffer."""
@@ -89,7 +92,18 @@ class QNTensor:
     def _allowed_keys(self):
         """Every charge-allowed key in lexicographic order: the leading indices run
         over all their sectors and the last one takes the sectors of the charge
-        that balances the flux."""
+        that balances the flux (cached per charge structure: the keys depend only
+        on the sector charges, the arrows and the flux)."""
+        ck = (tuple(tuple(ix.q) for ix in self.indices), tuple(self.dirs), self.flux)
+        keys = _KEYS_CACHE.get(ck)
+        if keys is None:
+            keys = self._enumerate_keys()
+            if len(_KEYS_CACHE) > 4096:
+                _KEYS_CACHE.clear()
+            _KEYS_CACHE[ck] = keys
+        return keys
+
+    def _enumerate_keys(self):
         *lead, last = self.indices
         *dl, d_last = self.dirs
         keys = []
