@@ -290,9 +290,10 @@ void vec_lincomb(nsmi_ctx* ctx, double* out, const double* const* V, const doubl
 void vec_axpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div);
 void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div);
 // r += f <q, r> q (sum conj(q) r for complex) in two launches, the dot product also
-// left in slot (and slot + 1): one modified Gram-Schmidt step with the coefficient on the device
-void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f);
-void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f);
+// left in slot (and slot + 1): one modified Gram-Schmidt step with the coefficient on the
+// device; with norm_slot >= 0 the squared norm of the new r goes to norm_slot as well
+void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1);
+void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1);
 // read result slots [0, count) to host (synchronises)
 void read_slots(nsmi_ctx* ctx, int count, double* out);
 void vec_to_f32(nsmi_ctx* ctx, float* y, const double* x, int64_t n);

@@ -69,12 +69,13 @@ struct KOps {
       vec_dot_dev(ctx, a, b, n, slot);
   }
   void norm2_dev(const double* a, int slot) const { vec_dot_dev(ctx, a, a, is_c ? 2 * n : n, slot); }
-  // r += f <q, r> q, the dot product also left in slot: one Gram-Schmidt step
-  void dot_axpy(const double* q, double* r, int slot, double f) const {
+  // r += f <q, r> q, the dot product also left in slot (and |r|^2 in norm_slot when
+  // given): one Gram-Schmidt step
+  void dot_axpy(const double* q, double* r, int slot, double f, int norm_slot = -1) const {
     if (is_c)
-      vec_zdot_axpy(ctx, q, r, n, slot, f);
+      vec_zdot_axpy(ctx, q, r, n, slot, f, norm_slot);
     else
-      vec_dot_axpy(ctx, q, r, n, slot, f);
+      vec_dot_axpy(ctx, q, r, n, slot, f, norm_slot);
   }
   // y += f * c x with c the (complex) slot value, or y += f * (Re c / div) x when div != 0
   void axpy_dc(double* y, const double* x, int slot, double f, double div = 0.0) const {
@@ -114,18 +115,14 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
   ops.scaled_copy(vk, r, 1.0 / beta);
   heff_apply_dev(ops.ctx, op, vk, r);
   ops.axpy(r, vprev, cplx(-beta, 0.0));
-  ops.dot_axpy(vk, r, kSlotDot, -1.0);
-  ops.norm2_dev(r, kSlotN2a);
+  ops.dot_axpy(vk, r, kSlotDot, -1.0, kSlotN2a);
   read_slots(ops.ctx, 3, sl);
   double a = sl[0];
   double b = std::sqrt(std::max(sl[kSlotN2a], 0.0));
   double nold = std::sqrt(b * b + a * a + beta * beta);
   while (kEps < b && b < eta * nold) {
     nold = b;
-    for (int j = 0; j <= k; ++j) {
-      ops.dot_axpy(V + nd * j, r, kSlotDot, -1.0);
-    }
-    ops.norm2_dev(r, kSlotN2a);
+    for (int j = 0; j <= k; ++j) ops.dot_axpy(V + nd * j, r, kSlotDot, -1.0, j == k ? kSlotN2a : -1);
     read_slots(ops.ctx, 3, sl);
     a += sl[0];  // the last coefficient (against v_k)
     b = std::sqrt(std::max(sl[kSlotN2a], 0.0));
@@ -152,8 +149,7 @@ void lanczos_start(const KOps& ops, nsmi_heff* op, const double* x0, double beta
   double beta = std::sqrt(std::max(sl[kSlotN2b], 0.0));
   while (kEps < beta && beta < eta * beta_old) {
     beta_old = beta;
-    ops.dot_axpy(V, r, kSlotDot, -1.0);
-    ops.norm2_dev(r, kSlotN2b);
+    ops.dot_axpy(V, r, kSlotDot, -1.0, kSlotN2b);
     read_slots(ops.ctx, 4, sl);
     alpha += sl[0];
     beta = std::sqrt(std::max(sl[kSlotN2b], 0.0));

@@ -161,6 +161,25 @@ __device__ inline double partials_sum(const double* __restrict__ partials, int n
   return total;
 }
 
+// The same step with the squared norm of the updated y summed on the way: the grid
+// and the grid-stride order are those of the separate norm reduction (vec_dot_dev of
+// y with itself), so the partials -- and the norm -- are the ones it would produce.
+__global__ void k_axpy_dp_norm(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                               const double* __restrict__ partials, int np, double f, double* __restrict__ slot,
+                               double* __restrict__ npartials) {
+  const double c = partials_sum(partials, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) slot[0] = c;
+  const double a = f * c;
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double t = y[i] + a * x[i];
+    y[i] = t;
+    acc += t * t;
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) npartials[blockIdx.x] = s;
+}
+
 __global__ void k_axpy_dp(double* __restrict__ y, const double* __restrict__ x, int64_t n,
                           const double* __restrict__ partials, int np, double f, double* __restrict__ slot) {
   const double c = partials_sum(partials, np);
@@ -214,6 +233,28 @@ __global__ void k_zdot_partial(const double* __restrict__ x, const double* __res
     partials[blockIdx.x] = sr;
     partials[kPartials + blockIdx.x] = si;
   }
+}
+
+// complex form over the 2n doubles of y in the norm reduction's order: each double
+// takes its component of (ar + i ai) x
+__global__ void k_zaxpy_dp_norm(double* __restrict__ y, const double* __restrict__ x, int64_t n,
+                                const double* __restrict__ partials, int np, double f, double* __restrict__ slot,
+                                double* __restrict__ npartials) {
+  const double cr = partials_sum(partials, np);
+  const double ci = partials_sum(partials + kPartials, np);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    slot[0] = cr;
+    slot[1] = ci;
+  }
+  const double ar = f * cr, ai = f * ci;
+  double acc = 0.0;
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < 2 * n; j += int64_t(gridDim.x) * blockDim.x) {
+    const double t = (j & 1) ? y[j] + (ar * x[j] + ai * x[j - 1]) : y[j] + (ar * x[j] - ai * x[j + 1]);
+    y[j] = t;
+    acc += t * t;
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) npartials[blockIdx.x] = s;
 }
 
 __global__ void k_zaxpy_dp(double* __restrict__ y, const double* __restrict__ x, int64_t n,
@@ -326,25 +367,45 @@ void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot
   NSMI_LAUNCH_CHECK();
 }
 
-void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f) {
-  require(slot >= 0 && slot < 64, "slot out of range");
+void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot) {
+  require(slot >= 0 && slot < 64 && norm_slot < 64, "slot out of range");
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 40.0 * n);
   const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
   hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
   NSMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_axpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
-                     ctx->d_scal + slot);
+  if (norm_slot < 0) {
+    hipLaunchKernelGGL(k_axpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
+                       ctx->d_scal + slot);
+    NSMI_LAUNCH_CHECK();
+    return;
+  }
+  double* np2 = ctx->d_partials + 2 * kPartials;
+  hipLaunchKernelGGL(k_axpy_dp_norm, dim3(g), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
+                     ctx->d_scal + slot, np2);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, np2, g, ctx->d_scal + norm_slot);
   NSMI_LAUNCH_CHECK();
 }
 
-void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f) {
-  require(slot >= 0 && slot + 1 < 64, "slot out of range");
+void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot) {
+  require(slot >= 0 && slot + 1 < 64 && norm_slot < 64, "slot out of range");
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 80.0 * n);
   const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
   hipLaunchKernelGGL(k_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
   NSMI_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_zaxpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
-                     ctx->d_scal + slot);
+  if (norm_slot < 0) {
+    hipLaunchKernelGGL(k_zaxpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g,
+                       f, ctx->d_scal + slot);
+    NSMI_LAUNCH_CHECK();
+    return;
+  }
+  // the norm reduction's grid over the 2n doubles
+  const int g2 = 2 * n > 0 ? std::min(grid_for(2 * n, 4), kPartials) : 1;
+  double* np2 = ctx->d_partials + 2 * kPartials;
+  hipLaunchKernelGGL(k_zaxpy_dp_norm, dim3(g2), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
+                     ctx->d_scal + slot, np2);
+  NSMI_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kThreads), 0, ctx->stream, np2, g2, ctx->d_scal + norm_slot);
   NSMI_LAUNCH_CHECK();
 }
 
