@@ -291,9 +291,12 @@ void vec_axpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot,
 void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot, double f, double div);
 // r += f <q, r> q (sum conj(q) r for complex) in two launches, the dot product also
 // left in slot (and slot + 1): one modified Gram-Schmidt step with the coefficient on the
-// device; with norm_slot >= 0 the squared norm of the new r goes to norm_slot as well
-void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1);
-void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1);
+// device; with norm_slot >= 0 the squared norm of the new r goes to norm_slot as well,
+// and with prev set r += cprev * prev first (in the same pass as the dot product)
+void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1,
+                  const double* prev = nullptr, double cprev = 0.0);
+void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot = -1,
+                   const double* prev = nullptr, double cprev = 0.0);
 // read result slots [0, count) to host (synchronises)
 void read_slots(nsmi_ctx* ctx, int count, double* out);
 void vec_to_f32(nsmi_ctx* ctx, float* y, const double* x, int64_t n);

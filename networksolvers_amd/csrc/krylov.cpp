@@ -71,11 +71,13 @@ struct KOps {
   void norm2_dev(const double* a, int slot) const { vec_dot_dev(ctx, a, a, is_c ? 2 * n : n, slot); }
   // r += f <q, r> q, the dot product also left in slot (and |r|^2 in norm_slot when
   // given): one Gram-Schmidt step
-  void dot_axpy(const double* q, double* r, int slot, double f, int norm_slot = -1) const {
+  // (with prev set, r += cprev * prev first, in the same pass)
+  void dot_axpy(const double* q, double* r, int slot, double f, int norm_slot = -1, const double* prev = nullptr,
+                double cprev = 0.0) const {
     if (is_c)
-      vec_zdot_axpy(ctx, q, r, n, slot, f, norm_slot);
+      vec_zdot_axpy(ctx, q, r, n, slot, f, norm_slot, prev, cprev);
     else
-      vec_dot_axpy(ctx, q, r, n, slot, f, norm_slot);
+      vec_dot_axpy(ctx, q, r, n, slot, f, norm_slot, prev, cprev);
   }
   // y += f * c x with c the (complex) slot value, or y += f * (Re c / div) x when div != 0
   void axpy_dc(double* y, const double* x, int slot, double f, double div = 0.0) const {
@@ -114,8 +116,7 @@ void lanczos_expand(const KOps& ops, nsmi_heff* op, double* V, int k, double* r,
   double sl[4];
   ops.scaled_copy(vk, r, 1.0 / beta);
   heff_apply_dev(ops.ctx, op, vk, r);
-  ops.axpy(r, vprev, cplx(-beta, 0.0));
-  ops.dot_axpy(vk, r, kSlotDot, -1.0, kSlotN2a);
+  ops.dot_axpy(vk, r, kSlotDot, -1.0, kSlotN2a, vprev, -beta);  // r -= beta v_{k-1}, then Gram-Schmidt
   read_slots(ops.ctx, 3, sl);
   double a = sl[0];
   double b = std::sqrt(std::max(sl[kSlotN2a], 0.0));

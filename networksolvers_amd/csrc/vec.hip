@@ -147,6 +147,21 @@ __global__ void k_nonfinite_partial(const double* __restrict__ x, int64_t n, dou
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// r += c p, then the partial sums of <q, r> over the updated r (same grid and order
+// as k_dot_partial), in one pass: the three-term recurrence step fused into the first
+// Gram-Schmidt step's reduction
+__global__ void k_axpy_dot_partial(double* __restrict__ r, const double* __restrict__ p, double c,
+                                   const double* __restrict__ q, int64_t n, double* __restrict__ partials) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double t = r[i] + c * p[i];
+    r[i] = t;
+    acc += q[i] * t;
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // The finishing pass of a dot product folded into the axpy that consumes it: every
 // block adds the partials in the order k_finish uses, so all blocks -- and the slot
 // block 0 leaves for the host -- see the same coefficient c; then y += f c x.
@@ -237,6 +252,27 @@ __global__ void k_zdot_partial(const double* __restrict__ x, const double* __res
 
 // complex form over the 2n doubles of y in the norm reduction's order: each double
 // takes its component of (ar + i ai) x
+// complex form: r += c p (c real), then sum conj(q) r partials as k_zdot_partial
+__global__ void k_zaxpy_zdot_partial(double* __restrict__ r, const double* __restrict__ p, double c,
+                                     const double* __restrict__ q, int64_t n, double* __restrict__ partials) {
+  double re = 0.0, im = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double yr = r[2 * i] + c * p[2 * i];
+    const double yi = r[2 * i + 1] + c * p[2 * i + 1];
+    r[2 * i] = yr;
+    r[2 * i + 1] = yi;
+    const double xr = q[2 * i], xi = q[2 * i + 1];
+    re += xr * yr + xi * yi;
+    im += xr * yi - xi * yr;
+  }
+  const double sr = block_sum(re);
+  const double si = block_sum(im);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = sr;
+    partials[kPartials + blockIdx.x] = si;
+  }
+}
+
 __global__ void k_zaxpy_dp_norm(double* __restrict__ y, const double* __restrict__ x, int64_t n,
                                 const double* __restrict__ partials, int np, double f, double* __restrict__ slot,
                                 double* __restrict__ npartials) {
@@ -367,11 +403,16 @@ void vec_zaxpy_dc(nsmi_ctx* ctx, double* y, const double* x, int64_t n, int slot
   NSMI_LAUNCH_CHECK();
 }
 
-void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot) {
+void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot,
+                  const double* prev, double cprev) {
   require(slot >= 0 && slot < 64 && norm_slot < 64, "slot out of range");
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 40.0 * n);
   const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
-  hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
+  if (prev)
+    hipLaunchKernelGGL(k_axpy_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, r, prev, cprev, q, n,
+                       ctx->d_partials);
+  else
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
   NSMI_LAUNCH_CHECK();
   if (norm_slot < 0) {
     hipLaunchKernelGGL(k_axpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g, f,
@@ -387,11 +428,16 @@ void vec_dot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot
   NSMI_LAUNCH_CHECK();
 }
 
-void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot) {
+void vec_zdot_axpy(nsmi_ctx* ctx, const double* q, double* r, int64_t n, int slot, double f, int norm_slot,
+                   const double* prev, double cprev) {
   require(slot >= 0 && slot + 1 < 64 && norm_slot < 64, "slot out of range");
   ProfScope ps(ctx, NSMI_K_KRYLOV, 0.0, 80.0 * n);
   const int g = n > 0 ? std::min(grid_for(n, 4), kPartials) : 1;
-  hipLaunchKernelGGL(k_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
+  if (prev)
+    hipLaunchKernelGGL(k_zaxpy_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, r, prev, cprev, q, n,
+                       ctx->d_partials);
+  else
+    hipLaunchKernelGGL(k_zdot_partial, dim3(g), dim3(kThreads), 0, ctx->stream, q, r, n, ctx->d_partials);
   NSMI_LAUNCH_CHECK();
   if (norm_slot < 0) {
     hipLaunchKernelGGL(k_zaxpy_dp, dim3(grid_for(n)), dim3(kThreads), 0, ctx->stream, r, q, n, ctx->d_partials, g,
