@@ -195,30 +195,39 @@ class _Program:
         self.scratch.append(int(n))
         return None  # resolved in build (ids after the operands)
 
+    # The item tables are kept flat (7 ints per GEMM item and segment, 5 per linear
+    # combination item, 2 per term, 8 per copy) so that build converts them at once.
     def gemm(self, ta, tb, items):
         """items: [(c_buf, c_off, ldc, M, N, [(a_buf, a_off, lda, b_buf, b_off, ldb, K, alpha), ...])]"""
-        first = len(self.gitems)
+        gi, sgl, al = self.gitems, self.segs, self.alpha
+        first = len(gi) // 7
+        nseg = len(al)
         for c_buf, c_off, ldc, M, N, sg in items:
-            self.gitems.append((c_buf, c_off, ldc, M, N, len(self.segs), len(sg)))
-            for a_buf, a_off, lda, b_buf, b_off, ldb, K, al in sg:
-                self.segs.append((a_buf, a_off, lda, b_buf, b_off, ldb, K))
-                self.alpha.append(al)
+            gi += (c_buf, c_off, ldc, M, N, nseg, len(sg))
+            for seg in sg:
+                sgl += seg[:7]
+                al.append(seg[7])
+            nseg += len(sg)
         self.steps.append((0, int(ta), int(tb), first, len(items), 0))
 
     def lincomb(self, items):
         """items: [(out_buf, out_off, n, [(src_buf, src_off, coeff), ...])]"""
-        first = len(self.litems)
+        li, te, co = self.litems, self.terms, self.coeff
+        first = len(li) // 5
+        nt = len(co)
         for ob, oo, n, tl in items:
-            self.litems.append((ob, oo, n, len(self.terms), len(tl)))
+            li += (ob, oo, n, nt, len(tl))
             for sb, so, c in tl:
-                self.terms.append((sb, so))
-                self.coeff.append(c)
+                te += (sb, so)
+                co.append(c)
+            nt += len(tl)
         self.steps.append((1, 0, 0, first, len(items), 0))
 
     def copy(self, items):
         """items: [(src_buf, src_off, src_ld, dst_buf, dst_off, dst_ld, rows, cols)]"""
-        first = len(self.citems)
-        self.citems.extend(items)
+        first = len(self.citems) // 8
+        for it in items:
+            self.citems += it
         self.steps.append((2, 0, 0, first, len(items), 0))
 
     def sid(self, j):
